@@ -1,0 +1,208 @@
+/*
+ * geon_hip.h — C ABI of the MI355X (gfx950) geodesic engine.
+ *
+ * Plain C: fixed-width integers, doubles and opaque pointers only. Every
+ * entry point returns an int status (GEON_OK == 0, negative on error) and
+ * never throws or aborts across the boundary.
+ *
+ * Reference interfaces each entry point replaces (paths in Arth-Singh/geon):
+ *   geon_metric_default      Schwarzschild::new / Alcubierre::default /
+ *                            MorrisThorne::default / Minkowski
+ *                            (src/metric/schwarzschild.rs:23-29,
+ *                             src/metric/alcubierre.rs:21-30,
+ *                             src/metric/morris_thorne.rs:14-19,
+ *                             src/metric/minkowski.rs:8)
+ *   geon_tetrad              Camera::tetrad            (src/camera.rs:47-97)
+ *   geon_trace_rays[_dev]    geodesic::trace, batched  (src/geodesic.rs:62-117)
+ *   geon_render[_dev]        render::render            (src/render.rs:55-165)
+ *   geon_curvature[_dev]     curvature::{riemann,ricci,ricci_scalar,einstein,
+ *                            stress_energy}            (src/curvature.rs:48-126)
+ *   geon_shadow_bisect       bin/validate_shadow.rs:20-95 (is_captured +
+ *                            bisection on the launch angle)
+ *   geon_tonemap_u8          render::write_png tonemap (src/render.rs:169-180)
+ *
+ * Memory: "_dev" entry points take device pointers (HBM) and a hipStream_t
+ * passed as void*; they enqueue work and return without synchronising.
+ * The other entry points take caller-owned host buffers, copy in and out,
+ * and synchronise. The context owns every device allocation it makes.
+ *
+ * Threading: a context is bound to one device and used by one host thread
+ * at a time. Separate contexts may be used concurrently.
+ */
+#ifndef GEON_HIP_H
+#define GEON_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define GEON_OK                 0
+#define GEON_E_INVALID         -1  /* bad argument / size                          */
+#define GEON_E_NOT_TIMELIKE    -2  /* camera.rs:53  observer has g_tt >= 0          */
+#define GEON_E_GRAM_SCHMIDT    -3  /* camera.rs:85  non-spacelike tetrad leg        */
+#define GEON_E_SINGULAR        -4  /* types.rs:39   metric not invertible           */
+#define GEON_E_HIP             -5  /* a HIP runtime call failed                     */
+#define GEON_E_NOMEM           -6  /* device allocation failed                      */
+#define GEON_E_ADMISSION       -7  /* request does not fit the HBM budget           */
+#define GEON_E_NO_MODULE       -8  /* gfx950 code object missing / not loadable     */
+#define GEON_E_UNSUPPORTED     -9  /* combination not provided by this build        */
+#define GEON_E_NOT_FOUND      -10  /* unknown ticket                                */
+#define GEON_E_PENDING        -11  /* request still running                         */
+
+/* ---- metric ---------------------------------------------------------- */
+#define GEON_METRIC_MINKOWSKI      0
+#define GEON_METRIC_SCHWARZSCHILD  1
+#define GEON_METRIC_ALCUBIERRE     2
+#define GEON_METRIC_MORRIS_THORNE  3
+
+/*
+ * kind-specific parameters p[]:
+ *   SCHWARZSCHILD  p[0] = mass
+ *   ALCUBIERRE     p[0] = bubble_radius R, p[1] = bubble_thickness s,
+ *                  p[2] = ship_velocity v, p[3] = ship_x
+ *   MORRIS_THORNE  p[0] = throat_radius b0
+ * Termination (metric::terminate of each reference metric):
+ *   SCHWARZSCHILD  captured r < horizon_pad*2M, escaped r > escape_radius*M
+ *   MINKOWSKI,
+ *   ALCUBIERRE     escaped x²+y²+z² > escape_radius²
+ *   MORRIS_THORNE  escaped |l| > escape_radius
+ * Reference defaults: horizon_pad 1.001, escape_radius 1000, fd_step 1e-5.
+ */
+typedef struct geon_metric {
+    int32_t kind;
+    int32_t reserved;
+    double  p[6];
+    double  horizon_pad;
+    double  escape_radius;
+    double  fd_step;
+} geon_metric_t;
+
+/* Camera::new(observer, fov_y_deg, width, height)  (camera.rs:29-36) */
+typedef struct geon_camera {
+    double   observer[4];
+    double   fov_y_deg;
+    uint32_t width;
+    uint32_t height;
+} geon_camera_t;
+
+#define GEON_PREC_FP64   0   /* plain FP64 RK4                                   */
+#define GEON_PREC_MIXED  1   /* FP32 RK4 + null re-projection, FP64 for flagged  */
+
+/* RenderConfig (render.rs:17-41) plus the sky toggle (sky.rs:23) */
+typedef struct geon_render_cfg {
+    double   step;
+    uint32_t max_steps;
+    uint32_t samples_per_axis;
+    float    captured_rgb[3];
+    float    diverged_rgb[3];
+    uint32_t show_stars;
+    int32_t  precision;
+} geon_render_cfg_t;
+
+/* RenderStats (render.rs:45-51) plus counters the reference does not keep */
+typedef struct geon_render_stats {
+    uint64_t captured;          /* pixels, reference majority rule            */
+    uint64_t escaped;
+    uint64_t diverged;          /* includes max-steps pixels (reference rule) */
+    double   max_null_residual;
+    double   avg_steps;         /* mean over pixels of per-pixel integer avg  */
+    uint64_t max_steps_hit;     /* rays that exhausted max_steps              */
+    uint64_t ray_steps;         /* sum of steps_taken over every traced ray   */
+    uint64_t fp64_rays;         /* rays integrated in FP64 (mixed precision)  */
+    double   kernel_ms;         /* device time of the integration kernels     */
+} geon_render_stats_t;
+
+/* Per-ray outcome codes. The reference's Termination has three variants;
+ * max-steps is its Diverged with steps_taken == max_steps (geodesic.rs:107-116). */
+#define GEON_DIVERGED  0
+#define GEON_CAPTURED  1
+#define GEON_ESCAPED   2
+#define GEON_MAXSTEPS  3
+
+/* curvature outputs (row-major, per point) */
+#define GEON_CURV_RIEMANN   0   /* 256 doubles R^ρ_σμν as [ρ][σ][μ][ν]  */
+#define GEON_CURV_RICCI     1   /* 16 doubles R_μν                       */
+#define GEON_CURV_EINSTEIN  2   /* 16 doubles G_μν                       */
+#define GEON_CURV_STRESS    3   /* 16 doubles T_μν = G_μν/(8π)           */
+#define GEON_CURV_SCALAR    4   /* 1 double  R                           */
+
+typedef struct geon_ctx geon_ctx_t;
+
+const char* geon_strerror(int status);
+int  geon_version(void);
+
+int  geon_metric_default(int32_t kind, geon_metric_t* out);
+
+/* code_object: path of geon_kernels.hsaco, or NULL for the copy installed
+ * beside libgeon_hip.so. */
+int  geon_ctx_create(int device, const char* code_object, geon_ctx_t** out);
+void geon_ctx_destroy(geon_ctx_t* ctx);
+
+/* Host-side Gram–Schmidt tetrad, rows e_(a)^μ, 16 doubles row-major. */
+int  geon_tetrad(const geon_metric_t* metric, const double observer[4],
+                 double tetrad_out[16]);
+
+/* Camera pixel -> initial state (camera.rs:113-160), host side. */
+int  geon_ray_state(const geon_camera_t* cam, const double tetrad[16],
+                    uint32_t i, uint32_t j, double du, double dv, double y_out[8]);
+
+/*
+ * Trace n rays from initial states y0 (SoA: 8 planes of n doubles).
+ * Outputs (any may be NULL): cls[n] (GEON_* codes), steps[n],
+ * y_final (SoA 8×n), null_resid[n], dir (SoA 3×n, asymptotic direction of
+ * escaped rays, zeros otherwise).
+ */
+int  geon_trace_rays(geon_ctx_t* ctx, const geon_metric_t* metric,
+                     const double* y0_soa, size_t n, double step, uint32_t max_steps,
+                     int32_t* cls, uint32_t* steps, double* y_final_soa,
+                     double* null_resid, double* dir_soa);
+int  geon_trace_rays_dev(geon_ctx_t* ctx, const geon_metric_t* metric,
+                         const double* y0_soa, size_t n, double step, uint32_t max_steps,
+                         int32_t* cls, uint32_t* steps, double* y_final_soa,
+                         double* null_resid, double* dir_soa, void* stream);
+
+/*
+ * Render a frame. rgb: 3·W·H floats (row-major, top-left origin, as
+ * render.rs:146-152); cls_per_pixel: W·H GEON_* codes of the pixel's
+ * representative class (render.rs:129-135; GEON_MAXSTEPS when every
+ * diverged sub-ray ran out of steps); dir_per_pixel: SoA 3×(W·H), the
+ * asymptotic direction of the pixel's first escaped sub-ray. Any output
+ * pointer may be NULL.
+ */
+int  geon_render(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_camera_t* cam,
+                 const geon_render_cfg_t* cfg, float* rgb, int32_t* cls_per_pixel,
+                 double* dir_per_pixel, geon_render_stats_t* stats);
+/* Device variant: outputs are device pointers; stats (host) is filled after
+ * an internal synchronisation of `stream` when non-NULL. */
+int  geon_render_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_camera_t* cam,
+                     const geon_render_cfg_t* cfg, float* rgb, int32_t* cls_per_pixel,
+                     double* dir_per_pixel, geon_render_stats_t* stats, void* stream);
+
+/* Curvature at n points x (SoA 4×n). out: n×{256|16|1} doubles. */
+int  geon_curvature(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa,
+                    size_t n, int32_t what, double* out);
+int  geon_curvature_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa,
+                        size_t n, int32_t what, double* out, void* stream);
+
+/*
+ * Shadow bisection (validate_shadow.rs): for each of n independent
+ * problems k (Schwarzschild mass[k], observer radius r_obs[k]), bisect
+ * `iters` times on the equatorial launch angle α ∈ [0, π/2 − 1e-3] using
+ * trace(step, max_steps) capture as the predicate, and return
+ * alpha_c[k] = ½(lo+hi) and b_c[k] = r_obs·sin α_c/√(1−2M/r_obs).
+ */
+int  geon_shadow_bisect(geon_ctx_t* ctx, const double* mass, const double* r_obs,
+                        size_t n, double step, uint32_t max_steps, uint32_t iters,
+                        double* alpha_c, double* b_c);
+
+/* write_png's per-channel map: clamp(0,1)^(1/2.2)·255, rounded (render.rs:175-179). */
+int  geon_tonemap_u8(const float* rgb, size_t n, uint8_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GEON_HIP_H */
