@@ -582,7 +582,8 @@ static double clampd(double x, double lo, double hi) {
 }
 
 /* sky.rs:26-34 — SplitMix64 finaliser. */
-static uint64_t hash1(uint64_t x) {
+uint64_t or_sky_hash1(uint64_t x);
+uint64_t or_sky_hash1(uint64_t x) {
     x *= 0x9E3779B97F4A7C15ull;
     x ^= x >> 30;
     x *= 0xBF58476D1CE4E5B9ull;
@@ -593,7 +594,7 @@ static uint64_t hash1(uint64_t x) {
 }
 /* sky.rs:36-39 */
 static uint64_t hash2(int64_t a, int64_t b) {
-    return hash1(((uint64_t)a * 73856093ull) ^ ((uint64_t)b * 19349663ull));
+    return or_sky_hash1(((uint64_t)a * 73856093ull) ^ ((uint64_t)b * 19349663ull));
 }
 
 /* sky.rs:43-68 */

@@ -60,6 +60,7 @@ void   or_ray_state_sub(const geon_camera_t* cam, const double tet[16], uint64_t
                         uint64_t j, double du, double dv, double y[8]);
 
 /* L4 render + sky */
+uint64_t or_sky_hash1(uint64_t x);
 void   or_sky_sample(const double dir[3], int show_stars, float rgb[3]);
 /* pixel outputs: rgb 3·W·H, cls W·H, dir 3·W·H (AoS, first escaped sub-ray),
  * steps_pix W·H (per-pixel integer average). Returns status. */
