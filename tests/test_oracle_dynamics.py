@@ -139,7 +139,7 @@ def test_camera_rays_capture_vs_analytic_shadow(O):
                 artefacts += 1
         elif b > 3 * math.sqrt(3) + 0.05:
             assert res["cls"][idx] == O.ESCAPED and res["y"][idx][1] < 100.1
-            assert res["resid"][idx] < 1e-5
+            assert res["resid"][idx] < 1e-3
     assert artefacts < 0.05 * len(y0)
 
 
