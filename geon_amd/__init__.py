@@ -1,0 +1,299 @@
+"""geon_amd — MI355X (gfx950) geodesic ray tracer and curvature engine.
+
+Python face of the C ABI in include/geon_hip.h, named after the reference
+crate's public surface (Arth-Singh/geon src/lib.rs:18-30): metric types with
+the reference field names and defaults, Camera, RenderConfig, RenderStats,
+render, trace, and the curvature functions. All compute runs in the HIP
+code object; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _abi
+from ._abi import (ALCUBIERRE, CAPTURED, CURV_EINSTEIN, CURV_RICCI, CURV_RIEMANN, CURV_SCALAR,
+                   CURV_STRESS, DIVERGED, ESCAPED, MAXSTEPS, MINKOWSKI, MORRIS_THORNE,
+                   PREC_FP64, PREC_MIXED, SCHWARZSCHILD, GeonError, check, lib)
+
+__all__ = ["Minkowski", "Schwarzschild", "Alcubierre", "MorrisThorne", "Camera", "RenderConfig",
+           "Context", "render", "trace_rays", "riemann", "ricci", "ricci_scalar", "einstein",
+           "stress_energy", "nec_contraction", "shadow_bisect", "tetrad", "GeonError"]
+
+
+# ---- metrics (src/metric/*.rs) ----------------------------------------------
+
+@dataclass
+class _MetricBase:
+    horizon_pad: float = 1.001
+    escape_radius: float = 1000.0
+    fd_step: float = 1e-5
+
+    KIND = -1
+
+    def _params(self):
+        return []
+
+    def to_c(self) -> _abi.Metric:
+        m = _abi.Metric()
+        m.kind = self.KIND
+        for i, v in enumerate(self._params()):
+            m.p[i] = float(v)
+        m.horizon_pad = float(self.horizon_pad)
+        m.escape_radius = float(self.escape_radius)
+        m.fd_step = float(self.fd_step)
+        return m
+
+
+@dataclass
+class Minkowski(_MetricBase):
+    """minkowski.rs — flat Cartesian chart, escape x²+y²+z² > escape_radius²."""
+    KIND = MINKOWSKI
+
+
+@dataclass
+class Schwarzschild(_MetricBase):
+    """schwarzschild.rs — Schwarzschild::new(mass): horizon_pad 1.001, escape 1000 M."""
+    mass: float = 1.0
+    KIND = SCHWARZSCHILD
+
+    def _params(self):
+        return [self.mass]
+
+
+@dataclass
+class Alcubierre(_MetricBase):
+    """alcubierre.rs — Alcubierre::default(): R=2, s=0.5, v=2, ship_x=0."""
+    bubble_radius: float = 2.0
+    bubble_thickness: float = 0.5
+    ship_velocity: float = 2.0
+    ship_x: float = 0.0
+    KIND = ALCUBIERRE
+
+    def _params(self):
+        return [self.bubble_radius, self.bubble_thickness, self.ship_velocity, self.ship_x]
+
+
+@dataclass
+class MorrisThorne(_MetricBase):
+    """morris_thorne.rs — MorrisThorne::default(): b0 = 1."""
+    throat_radius: float = 1.0
+    KIND = MORRIS_THORNE
+
+    def _params(self):
+        return [self.throat_radius]
+
+
+def _cmetric(m):
+    if isinstance(m, _MetricBase):
+        return m.to_c()
+    if isinstance(m, _abi.Metric):
+        return m
+    return _abi.Metric.from_buffer_copy(bytes(m))  # same-layout structure from another binding
+
+
+# ---- camera / config (camera.rs, render.rs) ----------------------------------
+
+@dataclass
+class Camera:
+    """Camera::new(observer, fov_y_deg, width, height) (camera.rs:29-36)."""
+    observer: tuple
+    fov_y_deg: float
+    width: int
+    height: int
+
+    def to_c(self) -> _abi.Camera:
+        c = _abi.Camera()
+        for i in range(4):
+            c.observer[i] = float(self.observer[i])
+        c.fov_y_deg = float(self.fov_y_deg)
+        c.width, c.height = int(self.width), int(self.height)
+        return c
+
+
+@dataclass
+class RenderConfig:
+    """RenderConfig (render.rs:17-41) plus the sky toggle and precision."""
+    step: float = 0.05
+    max_steps: int = 20_000
+    captured_rgb: tuple = (0.0, 0.0, 0.0)
+    diverged_rgb: tuple = (1.0, 0.0, 1.0)
+    samples_per_axis: int = 1
+    show_stars: bool = True
+    precision: int = PREC_FP64
+
+    def to_c(self) -> _abi.RenderCfg:
+        r = _abi.RenderCfg()
+        r.step, r.max_steps, r.samples_per_axis = float(self.step), int(self.max_steps), \
+            int(self.samples_per_axis)
+        r.captured_rgb[:] = list(self.captured_rgb)
+        r.diverged_rgb[:] = list(self.diverged_rgb)
+        r.show_stars, r.precision = int(bool(self.show_stars)), int(self.precision)
+        return r
+
+
+# ---- context -------------------------------------------------------------------
+
+class Context:
+    """One device, one code object, one stream (geon_ctx_create)."""
+
+    def __init__(self, device: int = 0, code_object: str | None = None):
+        h = C.c_void_p()
+        path = (code_object or _abi.HSACO_PATH).encode()
+        check(lib().geon_ctx_create(device, path, C.byref(h)), "geon_ctx_create")
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            lib().geon_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+_default_ctx = None
+
+
+def default_context() -> Context:
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def tetrad(metric, observer):
+    """Camera::tetrad (camera.rs:47-97): rows e_(a)^μ."""
+    out = np.zeros(16)
+    obs = np.ascontiguousarray(observer, dtype=np.float64)
+    check(lib().geon_tetrad(C.byref(_cmetric(metric)), _ptr(obs), _ptr(out)), "geon_tetrad")
+    return out.reshape(4, 4)
+
+
+def trace_rays(metric, y0, step, max_steps, ctx: Context | None = None):
+    """Trace initial states y0 (n×8, AoS as numpy) on the GPU (geodesic::trace per ray).
+
+    Returns dict(cls, steps, y, resid, dir) with y n×8 and dir n×3.
+    """
+    ctx = ctx or default_context()
+    y0 = np.ascontiguousarray(np.asarray(y0, dtype=np.float64).reshape(-1, 8).T)  # SoA
+    n = y0.shape[1]
+    cls = np.zeros(n, np.int32)
+    steps = np.zeros(n, np.uint32)
+    yf = np.zeros((8, n))
+    resid = np.zeros(n)
+    d = np.zeros((3, n))
+    check(lib().geon_trace_rays(ctx.handle, C.byref(_cmetric(metric)), _ptr(y0), n, float(step),
+                                int(max_steps), _ptr(cls), _ptr(steps), _ptr(yf), _ptr(resid),
+                                _ptr(d)), "geon_trace_rays")
+    return dict(cls=cls, steps=steps, y=yf.T.copy(), resid=resid, dir=d.T.copy())
+
+
+def render(metric, camera: Camera, cfg: RenderConfig | None = None, ctx: Context | None = None):
+    """render::render (render.rs:55-165): returns (rgb W·H×3 float32, RenderStats dict,
+    per-pixel class W·H, per-pixel direction W·H×3)."""
+    ctx = ctx or default_context()
+    cfg = cfg or RenderConfig()
+    n = camera.width * camera.height
+    rgb = np.zeros((n, 3), np.float32)
+    cls = np.zeros(n, np.int32)
+    d = np.zeros((3, n))
+    st = _abi.RenderStats()
+    check(lib().geon_render(ctx.handle, C.byref(_cmetric(metric)), C.byref(camera.to_c()),
+                            C.byref(cfg.to_c()), _ptr(rgb), _ptr(cls), _ptr(d), C.byref(st)),
+          "geon_render")
+    return rgb, st.as_dict(), cls, d.T.copy()
+
+
+def render_dev(metric, camera: Camera, cfg: RenderConfig, rgb_ptr, cls_ptr, dir_ptr,
+               stream_ptr=None, want_stats=True, ctx: Context | None = None):
+    """Device-pointer render (geon_render_dev): outputs stay in HBM."""
+    ctx = ctx or default_context()
+    st = _abi.RenderStats()
+    check(lib().geon_render_dev(ctx.handle, C.byref(_cmetric(metric)), C.byref(camera.to_c()),
+                                C.byref(cfg.to_c()), C.c_void_p(rgb_ptr), C.c_void_p(cls_ptr),
+                                C.c_void_p(dir_ptr), C.byref(st) if want_stats else None,
+                                C.c_void_p(stream_ptr) if stream_ptr else None),
+          "geon_render_dev")
+    return st.as_dict()
+
+
+def curvature(metric, x, what, ctx: Context | None = None):
+    """Curvature at points x (n×4): what = CURV_RIEMANN|RICCI|EINSTEIN|STRESS|SCALAR."""
+    ctx = ctx or default_context()
+    xs = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 4).T)
+    n = xs.shape[1]
+    w = {CURV_RIEMANN: 256, CURV_SCALAR: 1}.get(what, 16)
+    out = np.zeros((n, w))
+    check(lib().geon_curvature(ctx.handle, C.byref(_cmetric(metric)), _ptr(xs), n, what,
+                               _ptr(out)), "geon_curvature")
+    return out
+
+
+def riemann(metric, x, ctx=None):
+    return curvature(metric, x, CURV_RIEMANN, ctx).reshape(-1, 4, 4, 4, 4)
+
+
+def ricci(metric, x, ctx=None):
+    return curvature(metric, x, CURV_RICCI, ctx).reshape(-1, 4, 4)
+
+
+def ricci_scalar(metric, x, ctx=None):
+    return curvature(metric, x, CURV_SCALAR, ctx).reshape(-1)
+
+
+def einstein(metric, x, ctx=None):
+    return curvature(metric, x, CURV_EINSTEIN, ctx).reshape(-1, 4, 4)
+
+
+def stress_energy(metric, x, ctx=None):
+    return curvature(metric, x, CURV_STRESS, ctx).reshape(-1, 4, 4)
+
+
+def nec_contraction(metric, x, k, ctx=None):
+    """T_μν k^μ k^ν (curvature.rs:127-136) at each point, summed in the reference order."""
+    T = stress_energy(metric, x, ctx)
+    k = np.asarray(k, dtype=np.float64).reshape(-1, 4)
+    k = np.broadcast_to(k, (T.shape[0], 4))
+    out = np.zeros(T.shape[0])
+    for mu in range(4):
+        for nu in range(4):
+            out = out + T[:, mu, nu] * k[:, mu] * k[:, nu]
+    return out
+
+
+def shadow_bisect(mass, r_obs, step=0.01, max_steps=200_000, iters=60, ctx=None):
+    """validate_shadow.rs bisection for each (mass[k], r_obs[k]) → (alpha_c, b_c)."""
+    ctx = ctx or default_context()
+    m = np.ascontiguousarray(np.atleast_1d(mass), dtype=np.float64)
+    r = np.ascontiguousarray(np.atleast_1d(r_obs), dtype=np.float64)
+    n = m.size
+    a = np.zeros(n)
+    b = np.zeros(n)
+    check(lib().geon_shadow_bisect(ctx.handle, _ptr(m), _ptr(r), n, float(step), int(max_steps),
+                                   int(iters), _ptr(a), _ptr(b)), "geon_shadow_bisect")
+    return a, b
+
+
+def tonemap_u8(rgb):
+    """write_png's per-channel map (render.rs:175-179)."""
+    px = np.ascontiguousarray(rgb, dtype=np.float32)
+    out = np.zeros(px.size, np.uint8)
+    check(lib().geon_tonemap_u8(_ptr(px), px.size, _ptr(out)), "geon_tonemap_u8")
+    return out.reshape(px.shape)

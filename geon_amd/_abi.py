@@ -1,0 +1,111 @@
+"""ctypes binding of include/geon_hip.h (libgeon_hip.so + geon_kernels.hsaco).
+
+There is no fallback: importing this module on a machine without the built
+library raises, and every call that reaches the GPU goes through the HIP code
+object loaded by the native runtime.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libgeon_hip.so")
+HSACO_PATH = os.path.join(LIB_DIR, "geon_kernels.hsaco")
+
+OK = 0
+E_INVALID, E_NOT_TIMELIKE, E_GRAM_SCHMIDT, E_SINGULAR, E_HIP = -1, -2, -3, -4, -5
+E_NOMEM, E_ADMISSION, E_NO_MODULE, E_UNSUPPORTED, E_NOT_FOUND, E_PENDING = -6, -7, -8, -9, -10, -11
+
+MINKOWSKI, SCHWARZSCHILD, ALCUBIERRE, MORRIS_THORNE = 0, 1, 2, 3
+DIVERGED, CAPTURED, ESCAPED, MAXSTEPS = 0, 1, 2, 3
+PREC_FP64, PREC_MIXED = 0, 1
+CURV_RIEMANN, CURV_RICCI, CURV_EINSTEIN, CURV_STRESS, CURV_SCALAR = 0, 1, 2, 3, 4
+
+
+class GeonError(RuntimeError):
+    def __init__(self, status: int, where: str, detail: str = ""):
+        self.status = status
+        super().__init__(f"{where}: {strerror(status)} ({status}){': ' + detail if detail else ''}")
+
+
+class Metric(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("p", C.c_double * 6),
+                ("horizon_pad", C.c_double), ("escape_radius", C.c_double),
+                ("fd_step", C.c_double)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("observer", C.c_double * 4), ("fov_y_deg", C.c_double),
+                ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+class RenderCfg(C.Structure):
+    _fields_ = [("step", C.c_double), ("max_steps", C.c_uint32),
+                ("samples_per_axis", C.c_uint32), ("captured_rgb", C.c_float * 3),
+                ("diverged_rgb", C.c_float * 3), ("show_stars", C.c_uint32),
+                ("precision", C.c_int32)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("captured", C.c_uint64), ("escaped", C.c_uint64), ("diverged", C.c_uint64),
+                ("max_null_residual", C.c_double), ("avg_steps", C.c_double),
+                ("max_steps_hit", C.c_uint64), ("ray_steps", C.c_uint64),
+                ("fp64_rays", C.c_uint64), ("kernel_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+# (name, restype, argtypes) of every entry point declared in include/geon_hip.h
+_vp, _dp, _i, _u32, _sz = C.c_void_p, C.POINTER(C.c_double), C.c_int, C.c_uint32, C.c_size_t
+SIGNATURES = [
+    ("geon_strerror", C.c_char_p, [_i]),
+    ("geon_last_error", C.c_char_p, []),
+    ("geon_version", _i, []),
+    ("geon_metric_default", _i, [C.c_int32, C.POINTER(Metric)]),
+    ("geon_ctx_create", _i, [_i, C.c_char_p, C.POINTER(_vp)]),
+    ("geon_ctx_destroy", None, [_vp]),
+    ("geon_tetrad", _i, [C.POINTER(Metric), _vp, _vp]),
+    ("geon_ray_state", _i, [C.POINTER(Camera), _vp, _u32, _u32, C.c_double, C.c_double, _vp]),
+    ("geon_trace_rays", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_double, _u32,
+                             _vp, _vp, _vp, _vp, _vp]),
+    ("geon_trace_rays_dev", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_double, _u32,
+                                 _vp, _vp, _vp, _vp, _vp, _vp]),
+    ("geon_render", _i, [_vp, C.POINTER(Metric), C.POINTER(Camera), C.POINTER(RenderCfg),
+                         _vp, _vp, _vp, C.POINTER(RenderStats)]),
+    ("geon_render_dev", _i, [_vp, C.POINTER(Metric), C.POINTER(Camera), C.POINTER(RenderCfg),
+                             _vp, _vp, _vp, C.POINTER(RenderStats), _vp]),
+    ("geon_curvature", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_int32, _vp]),
+    ("geon_curvature_dev", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_int32, _vp, _vp]),
+    ("geon_shadow_bisect", _i, [_vp, _vp, _vp, _sz, C.c_double, _u32, _u32, _vp, _vp]),
+    ("geon_tonemap_u8", _i, [_vp, _sz, _vp]),
+]
+
+_lib = None
+
+
+def lib():
+    """Load libgeon_hip.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m geon_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def strerror(status: int) -> str:
+    return lib().geon_strerror(status).decode()
+
+
+def check(status: int, where: str) -> None:
+    if status != OK:
+        detail = lib().geon_last_error().decode()
+        raise GeonError(status, where, detail)

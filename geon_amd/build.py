@@ -1,0 +1,68 @@
+"""Build the gfx950 code object and the host runtime in-tree.
+
+    python -m geon_amd.build
+
+Outputs (git-ignored, shipped beside the sources):
+  geon_amd/lib/geon_kernels.hsaco   hipcc --genco --offload-arch=gfx950
+  geon_amd/lib/libgeon_hip.so       host runtime + C ABI (g++, links libamdhip64)
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+
+# FP64 kernels keep the reference's rounding: no FMA contraction, no fast math.
+# Packed FP32 instructions are disabled for every HIP translation unit.
+HIP_FLAGS = ["--genco", "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+             "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
+              "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-result"]
+
+KERNEL_SRC = [os.path.join(CSRC, "kernels", "geon_kernels.hip")]
+KERNEL_DEPS = KERNEL_SRC + [os.path.join(CSRC, "kernels", "geon_device.hpp"),
+                            os.path.join(CSRC, "geon_dev_types.h")]
+HOST_SRC = [os.path.join(CSRC, "host", f) for f in ("geon_runtime.cpp", "geon_abi.cpp")]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_kernels(out=None):
+    out = out or os.path.join(LIB, "geon_kernels.hsaco")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, *KERNEL_SRC, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host(out=None):
+    out = out or os.path.join(LIB, "libgeon_hip.so")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, *HOST_FLAGS, *HOST_SRC, f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib",
+          "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all():
+    build_kernels()
+    build_host()
+
+
+if __name__ == "__main__":
+    build_all()
+    sys.exit(0)

@@ -1,0 +1,55 @@
+// geon_dev_types.h — plain structures shared by the host runtime and the
+// gfx950 kernels. Everything here is precomputed on the host (glibc libm,
+// same association order as the reference) so that constants the reference
+// recomputes per call, e.g. 2·tanh(R/s) or horizon_pad·2·M, carry the exact
+// values the reference sees.
+#pragma once
+#include <stdint.h>
+
+#define GEON_CONST_SLOTS 32   // ring of __constant__ frame slots
+
+struct geon_dmetric {
+    int32_t kind;
+    int32_t s_pow2;      // Alcubierre: thickness is a power of two → x/s == x*inv_s exactly
+    double  M;           // Schwarzschild mass
+    double  cap_r;       // horizon_pad * 2.0 * M             (schwarzschild.rs:114)
+    double  esc_r;       // escape_radius * M                 (schwarzschild.rs:116)
+    double  esc2;        // escape_radius²  (minkowski.rs:34, alcubierre.rs:70: 1e6)
+    double  esc_l;       // |l| escape (morris_thorne.rs:43)
+    double  R, s, inv_s, v, ship_x;
+    double  denom;       // 2.0 * tanh(R / s)                 (alcubierre.rs:42)
+    double  flat_rs;     // r_s above which f(r_s ± h) == 0 exactly
+    double  b0;          // throat radius
+    double  fd_h;        // metric/mod.rs:73-75
+    double  fd_two_h;    // 2.0 * h                           (metric/mod.rs:35)
+    double  fd_inv_2h;   // 1.0 / (2.0 * h)                   (curvature.rs:34)
+};
+
+struct geon_dframe {
+    geon_dmetric m;
+    double   tet[16];      // rows e_(a)^μ                   (camera.rs:47-97)
+    double   obs[4];
+    double   w_im, h_im;   // tan(fov/2)·aspect, tan(fov/2)   (camera.rs:119-121)
+    double   step;
+    double   inv_spa;      // 1.0 / spa                       (render.rs:71)
+    uint32_t W, H, spa, max_steps;
+    float    captured_rgb[3];
+    float    diverged_rgb[3];
+    uint32_t show_stars;
+    int32_t  precision;
+    // mixed precision: rays whose impact parameter lies within fp64_band of
+    // crit_b are integrated in FP64
+    double   crit_b;
+    double   fp64_band;
+    double   b_scale;      // b = b_scale * sin(alpha)
+    uint32_t reserved[4];
+};
+
+struct geon_dstats {
+    unsigned long long captured, escaped, diverged;
+    unsigned long long max_resid_bits;   // max over pixels, as the u64 image of a double ≥ 0
+    unsigned long long steps_pix_sum;    // Σ per-pixel integer-average steps
+    unsigned long long max_steps_hit;
+    unsigned long long ray_steps;
+    unsigned long long fp64_rays;
+};

@@ -1,0 +1,64 @@
+// geon_runtime.hpp — internal declarations of the host runtime.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+
+#include "../../../include/geon_hip.h"
+#include "../geon_dev_types.h"
+
+struct geon_ctx {
+    int device = 0;
+    hipModule_t mod = nullptr;
+    hipFunction_t fn_render[4] = {};
+    hipFunction_t fn_trace[4] = {};
+    hipFunction_t fn_curv[4] = {};
+    hipFunction_t fn_finalize = nullptr;
+    hipFunction_t fn_stats_clear = nullptr;
+    hipFunction_t fn_bisect = nullptr;
+    hipDeviceptr_t c_frames = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t slot_ev[GEON_CONST_SLOTS] = {};
+    bool slot_used[GEON_CONST_SLOTS] = {};
+    uint32_t next_slot = 0;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    geon_dstats* d_stats = nullptr;
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Buf> bufs;
+};
+
+namespace geon_host {
+
+extern thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+bool metric_valid(const geon_metric_t* m);
+void host_g(const geon_metric_t* m, const double x[4], double g[16]);
+int tetrad(const geon_metric_t* m, const double obs[4], double tet[16]);
+int make_dmetric(const geon_metric_t* m, geon_dmetric* d);
+
+int ctx_create(int device, const char* code_object, geon_ctx** out);
+void ctx_destroy(geon_ctx* c);
+int ctx_buffer(geon_ctx* c, const char* name, size_t bytes, void** p);
+int ctx_slot(geon_ctx* c, const geon_dframe* f, hipStream_t stream, uint32_t* slot);
+int ctx_slot_done(geon_ctx* c, uint32_t slot, hipStream_t stream);
+int launch(hipFunction_t f, uint64_t n_threads, uint32_t block, hipStream_t stream, void** args);
+
+}  // namespace geon_host
+
+#define GEON_TRY(expr)                 \
+    do {                               \
+        const int _st = (expr);        \
+        if (_st != GEON_OK) return _st; \
+    } while (0)
+
+#define GEON_HIP(expr, what)                                        \
+    do {                                                            \
+        const hipError_t _e = (expr);                               \
+        if (_e != hipSuccess) return geon_host::hip_fail(_e, what); \
+    } while (0)

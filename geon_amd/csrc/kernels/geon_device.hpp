@@ -1,0 +1,566 @@
+// geon_device.hpp — gfx950 device code for the per-ray geodesic path.
+//
+// Metric policies mirror the reference trait (src/metric/mod.rs:19-109):
+// each supplies g(x) and either analytic Γ (Schwarzschild, Minkowski) or
+// the trait's generic Γ from central-difference ∂g and g⁻¹ (Alcubierre,
+// Morris–Thorne). Every structural zero of g, ∂g, g⁻¹ and Γ is a
+// compile-time mask; the contractions walk the reference's loop order and
+// skip masked terms at compile time. Adding an exact zero never changes an
+// IEEE sum, so the skipped work leaves the results identical to the dense
+// reference loops (geodesic.rs:49-57, metric/mod.rs:55-66) while costing
+// only the nonzero terms. The FP64 build uses no FMA contraction.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <utility>
+
+#include "../geon_dev_types.h"
+
+#define GEON_D __device__ __forceinline__
+
+namespace geon {
+
+enum { DIVERGED = 0, CAPTURED = 1, ESCAPED = 2, MAXSTEPS = 3, RUNNING = -1 };
+
+// ---- compile-time loops --------------------------------------------------
+template <int N, class F, int... I>
+GEON_D void sfor_impl(F&& f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+GEON_D void sfor(F&& f) {
+    sfor_impl<N>(f, std::make_integer_sequence<int, N>{});
+}
+#define CI(v) decltype(v)::value
+
+GEON_D bool finite8(const double* y) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ok = ok && __builtin_isfinite(y[i]);
+    return ok;
+}
+
+// ---- generic Christoffel from FD ∂g (metric/mod.rs:25-69) -----------------
+// Policy MP provides: g_nz(μ,ν), dg_nz(α,μ,ν), ginv_nz(μ,σ) constexpr masks,
+// g(P, x, g[16]) filling the g_nz entries, ginv(P, g, gi[16]) filling the
+// ginv_nz entries.
+template <class MP>
+struct FD {
+    static constexpr bool S_nz(int s, int a, int b) {
+        return MP::dg_nz(a, s, b) || MP::dg_nz(b, s, a) || MP::dg_nz(s, a, b);
+    }
+    static constexpr bool gam_nz(int mu, int a, int b) {
+        return (MP::ginv_nz(mu, 0) && S_nz(0, a, b)) || (MP::ginv_nz(mu, 1) && S_nz(1, a, b)) ||
+               (MP::ginv_nz(mu, 2) && S_nz(2, a, b)) || (MP::ginv_nz(mu, 3) && S_nz(3, a, b));
+    }
+    static constexpr bool dir_nz(int a) {
+        bool any = false;
+        for (int m = 0; m < 4; ++m)
+            for (int n = 0; n < 4; ++n) any = any || MP::dg_nz(a, m, n);
+        return any;
+    }
+
+    // S(σ,α,β) = (∂_α g_σβ + ∂_β g_σα) − ∂_σ g_αβ with structural zeros dropped
+    template <int s, int a, int b>
+    GEON_D static double S(const double* dg) {
+        constexpr bool A = MP::dg_nz(a, s, b), B = MP::dg_nz(b, s, a), C = MP::dg_nz(s, a, b);
+        const double va = dg[a * 16 + s * 4 + b];
+        const double vb = dg[b * 16 + s * 4 + a];
+        const double vc = dg[s * 16 + a * 4 + b];
+        if constexpr (A && B && C) return (va + vb) - vc;
+        else if constexpr (A && B) return va + vb;
+        else if constexpr (A && C) return va - vc;
+        else if constexpr (B && C) return vb - vc;
+        else if constexpr (A) return va;
+        else if constexpr (B) return vb;
+        else return -vc;
+    }
+
+    // ∂g by central differences over the coordinates g depends on.
+    GEON_D static void dg(const geon_dmetric& P, const double* x, double* dg) {
+        sfor<4>([&](auto A) {
+            constexpr int a = CI(A);
+            if constexpr (dir_nz(a)) {
+                double xp[4] = {x[0], x[1], x[2], x[3]};
+                double xm[4] = {x[0], x[1], x[2], x[3]};
+                xp[a] += P.fd_h;
+                xm[a] -= P.fd_h;
+                double gp[16], gm[16];
+                MP::g(P, xp, gp);
+                MP::g(P, xm, gm);
+                sfor<4>([&](auto Mu) {
+                    sfor<4>([&](auto Nu) {
+                        constexpr int mu = CI(Mu), nu = CI(Nu);
+                        if constexpr (MP::dg_nz(a, mu, nu)) {
+                            if constexpr (nu < mu && MP::dg_nz(a, nu, mu))
+                                dg[a * 16 + mu * 4 + nu] = dg[a * 16 + nu * 4 + mu];  // symmetric g
+                            else
+                                dg[a * 16 + mu * 4 + nu] =
+                                    (gp[mu * 4 + nu] - gm[mu * 4 + nu]) / P.fd_two_h;
+                        }
+                    });
+                });
+            }
+        });
+    }
+
+    GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* gam) {
+        double g0[16], gi[16], dgv[64];
+        MP::g(P, x, g0);
+        MP::ginv(P, g0, gi);
+        dg(P, x, dgv);
+        sfor<4>([&](auto Mu) {
+            sfor<4>([&](auto A) {
+                sfor<4>([&](auto B) {
+                    constexpr int mu = CI(Mu), a = CI(A), b = CI(B);
+                    if constexpr (gam_nz(mu, a, b)) {
+                        if constexpr (b < a && gam_nz(mu, b, a)) {
+                            gam[mu * 16 + a * 4 + b] = gam[mu * 16 + b * 4 + a];
+                        } else {
+                            double s = 0.0;
+                            sfor<4>([&](auto Sg) {
+                                constexpr int sg = CI(Sg);
+                                if constexpr (MP::ginv_nz(mu, sg) && S_nz(sg, a, b))
+                                    s += gi[mu * 4 + sg] * S<sg, a, b>(dgv);
+                            });
+                            gam[mu * 16 + a * 4 + b] = 0.5 * s;
+                        }
+                    }
+                });
+            });
+        });
+    }
+};
+
+// dp^μ = −Σ_αβ Γ^μ_αβ p^α p^β (geodesic.rs:49-57), masked.
+template <class GP>
+GEON_D void contract(const double* gam, const double* p, double* dp) {
+    sfor<4>([&](auto Mu) {
+        constexpr int mu = CI(Mu);
+        double s = 0.0;
+        sfor<4>([&](auto A) {
+            sfor<4>([&](auto B) {
+                constexpr int a = CI(A), b = CI(B);
+                if constexpr (GP::gam_nz(mu, a, b)) s += gam[mu * 16 + a * 4 + b] * p[a] * p[b];
+            });
+        });
+        dp[mu] = -s;
+    });
+}
+
+// dot4 (types.rs:22-31) with g's structural zeros dropped.
+template <class MP>
+GEON_D double dot4(const double* g, const double* a, const double* b) {
+    double s = 0.0;
+    sfor<4>([&](auto I) {
+        sfor<4>([&](auto J) {
+            constexpr int i = CI(I), j = CI(J);
+            if constexpr (MP::g_nz(i, j)) s += g[i * 4 + j] * a[i] * b[j];
+        });
+    });
+    return s;
+}
+
+// ---- Minkowski (minkowski.rs) ---------------------------------------------
+struct Minkowski {
+    static constexpr bool g_nz(int m, int n) { return m == n; }
+    static constexpr bool ginv_nz(int m, int s) { return m == s; }
+    static constexpr bool gam_nz(int, int, int) { return false; }
+    GEON_D static void g(const geon_dmetric&, const double*, double* g) {
+        g[0] = -1.0; g[5] = 1.0; g[10] = 1.0; g[15] = 1.0;
+    }
+    GEON_D static void accel(const geon_dmetric&, const double*, double* dp) {
+        dp[0] = dp[1] = dp[2] = dp[3] = -0.0;
+    }
+    GEON_D static int terminate(const geon_dmetric& P, const double* x) {
+        double r2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+        return (r2 > P.esc2) ? ESCAPED : RUNNING;
+    }
+    GEON_D static void christoffel(const geon_dmetric&, const double*, double*) {}
+    static constexpr bool dep(int) { return false; }
+    GEON_D static void ginv_at(const geon_dmetric& P, const double* x, double* gi) { g(P, x, gi); }
+};
+
+// ---- Schwarzschild (schwarzschild.rs) -------------------------------------
+struct Schwarzschild {
+    static constexpr bool g_nz(int m, int n) { return m == n; }
+    static constexpr bool gam_nz(int mu, int a, int b) {
+        return (mu == 0 && ((a == 0 && b == 1) || (a == 1 && b == 0))) ||
+               (mu == 1 && a == b) ||
+               (mu == 2 && ((a == 1 && b == 2) || (a == 2 && b == 1) || (a == 3 && b == 3))) ||
+               (mu == 3 && ((a == 1 && b == 3) || (a == 3 && b == 1) || (a == 2 && b == 3) ||
+                            (a == 3 && b == 2)));
+    }
+    GEON_D static void g(const geon_dmetric& P, const double* x, double* g) {
+        const double r = x[1], m = P.M;
+        const double f = 1.0 - 2.0 * m / r;
+        const double st = sin(x[2]);
+        g[0] = -f;
+        g[5] = 1.0 / f;
+        g[10] = r * r;
+        g[15] = r * r * st * st;
+    }
+    // schwarzschild.rs:61-107
+    GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
+        const double r = x[1], m = P.M;
+        const double f = 1.0 - 2.0 * m / r;
+        double sin_t, cos_t;
+        sincos(x[2], &sin_t, &cos_t);
+        const double gtr = m / (r * r * f);
+        c[0 * 16 + 0 * 4 + 1] = gtr;
+        c[0 * 16 + 1 * 4 + 0] = gtr;
+        c[1 * 16 + 0 * 4 + 0] = m * f / (r * r);
+        c[1 * 16 + 1 * 4 + 1] = -m / (r * r * f);
+        c[1 * 16 + 2 * 4 + 2] = -(r - 2.0 * m);
+        c[1 * 16 + 3 * 4 + 3] = -(r - 2.0 * m) * sin_t * sin_t;
+        const double inv_r = 1.0 / r;
+        c[2 * 16 + 1 * 4 + 2] = inv_r;
+        c[2 * 16 + 2 * 4 + 1] = inv_r;
+        c[2 * 16 + 3 * 4 + 3] = -sin_t * cos_t;
+        c[3 * 16 + 1 * 4 + 3] = inv_r;
+        c[3 * 16 + 3 * 4 + 1] = inv_r;
+        const double cot = (fabs(sin_t) < 1e-8) ? 0.0 : cos_t / sin_t;
+        c[3 * 16 + 2 * 4 + 3] = cot;
+        c[3 * 16 + 3 * 4 + 2] = cot;
+    }
+    GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
+        double c[64];
+        christoffel(P, y, c);
+        contract<Schwarzschild>(c, y + 4, dp);
+    }
+    static constexpr bool dep(int d) { return d == 1 || d == 2; }
+    static constexpr bool ginv_nz(int m, int s) { return m == s; }
+    // schwarzschild.rs:47-59
+    GEON_D static void ginv_at(const geon_dmetric& P, const double* x, double* gi) {
+        const double r = x[1], m = P.M;
+        const double f = 1.0 - 2.0 * m / r;
+        const double st = sin(x[2]);
+        gi[0] = -1.0 / f;
+        gi[5] = f;
+        gi[10] = 1.0 / (r * r);
+        gi[15] = 1.0 / (r * r * st * st);
+    }
+    GEON_D static int terminate(const geon_dmetric& P, const double* x) {
+        const double r = x[1];
+        if (!__builtin_isfinite(r)) return DIVERGED;
+        if (r < P.cap_r) return CAPTURED;
+        if (r > P.esc_r) return ESCAPED;
+        return RUNNING;
+    }
+};
+
+// ---- Morris–Thorne (morris_thorne.rs), generic FD Γ -----------------------
+struct MorrisThorne {
+    static constexpr bool g_nz(int m, int n) { return m == n; }
+    static constexpr bool dg_nz(int a, int m, int n) {
+        return (a == 1 && m == n && (m == 2 || m == 3)) || (a == 2 && m == 3 && n == 3);
+    }
+    static constexpr bool ginv_nz(int m, int s) { return m == s; }
+    static constexpr bool gam_nz(int mu, int a, int b) { return FD<MorrisThorne>::gam_nz(mu, a, b); }
+    GEON_D static void g(const geon_dmetric& P, const double* x, double* g) {
+        const double l = x[1], b0 = P.b0;
+        const double r2 = b0 * b0 + l * l;
+        const double st = sin(x[2]);
+        g[0] = -1.0; g[5] = 1.0; g[10] = r2;
+        g[15] = r2 * st * st;
+    }
+    // Cofactor inverse of a diagonal matrix: c_ii = Π_{j≠i} m_jj in the
+    // cofactor scheme's order, det = m00·c00, g^ii = c_ii/det.
+    GEON_D static void ginv(const geon_dmetric&, const double* g, double* gi) {
+        const double m0 = g[0], m5 = g[5], m10 = g[10], m15 = g[15];
+        const double c00 = m5 * m10 * m15;
+        const double det = m0 * c00;
+        gi[0] = c00 / det;
+        gi[5] = m0 * m10 * m15 / det;
+        gi[10] = m0 * m5 * m15 / det;
+        gi[15] = m0 * m5 * m10 / det;
+    }
+    GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
+        FD<MorrisThorne>::christoffel(P, x, c);
+    }
+    GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
+        double c[64];
+        christoffel(P, y, c);
+        contract<MorrisThorne>(c, y + 4, dp);
+    }
+    static constexpr bool dep(int d) { return d == 1 || d == 2; }
+    GEON_D static void ginv_at(const geon_dmetric& P, const double* x, double* gi) {
+        double g0[16];
+        g(P, x, g0);
+        ginv(P, g0, gi);
+    }
+    GEON_D static int terminate(const geon_dmetric& P, const double* x) {
+        const double l = x[1];
+        if (!__builtin_isfinite(l)) return DIVERGED;
+        return (fabs(l) > P.esc_l) ? ESCAPED : RUNNING;
+    }
+};
+
+// ---- Alcubierre (alcubierre.rs), generic FD Γ -----------------------------
+struct Alcubierre {
+    static constexpr bool g_nz(int m, int n) {
+        return m == n || (m == 0 && n == 1) || (m == 1 && n == 0);
+    }
+    static constexpr bool dg_nz(int a, int m, int n) {
+        return a >= 1 && ((m == 0 && n == 0) || (m == 0 && n == 1) || (m == 1 && n == 0));
+    }
+    static constexpr bool ginv_nz(int m, int s) { return g_nz(m, s); }
+    static constexpr bool gam_nz(int mu, int a, int b) { return FD<Alcubierre>::gam_nz(mu, a, b); }
+
+    GEON_D static double rs(const geon_dmetric& P, const double* x) {
+        const double dx = x[1] - P.ship_x, dy = x[2], dz = x[3];
+        return sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    // alcubierre.rs:33-44 — denom = 2·tanh(R/s) precomputed on the host
+    GEON_D static double shape(const geon_dmetric& P, double r_s) {
+        double tp, tm;
+        if (P.s_pow2) {
+            tp = tanh((r_s + P.R) * P.inv_s);
+            tm = tanh((r_s - P.R) * P.inv_s);
+        } else {
+            tp = tanh((r_s + P.R) / P.s);
+            tm = tanh((r_s - P.R) / P.s);
+        }
+        return (tp - tm) / P.denom;
+    }
+    GEON_D static void g(const geon_dmetric& P, const double* x, double* g) {
+        const double v = P.v;
+        const double f = shape(P, rs(P, x));
+        g[0] = -1.0 + v * v * f * f;
+        g[1] = -v * f;
+        g[4] = -v * f;
+        g[5] = 1.0; g[10] = 1.0; g[15] = 1.0;
+    }
+    // Cofactor inverse of [[a,b],[b,1]] ⊕ I: det = a − b², g^00 = 1/det,
+    // g^01 = g^10 = −b/det, g^11 = a/det, g^22 = g^33 = det/det = 1.
+    GEON_D static void ginv(const geon_dmetric&, const double* g, double* gi) {
+        const double a = g[0], b = g[1];
+        const double det = a - b * b;
+        gi[0] = 1.0 / det;
+        gi[1] = -b / det;
+        gi[4] = gi[1];
+        gi[5] = a / det;
+        gi[10] = 1.0;
+        gi[15] = 1.0;
+    }
+    GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
+        FD<Alcubierre>::christoffel(P, x, c);
+    }
+    static constexpr bool dep(int d) { return d >= 1; }
+    GEON_D static void ginv_at(const geon_dmetric& P, const double* x, double* gi) {
+        double g0[16];
+        g(P, x, g0);
+        ginv(P, g0, gi);
+    }
+    GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
+        // Beyond flat_rs both tanh round to exactly 1 at every stencil point,
+        // so f and every ∂g vanish and Γ is exactly zero.
+        if (rs(P, y) > P.flat_rs) {
+            dp[0] = dp[1] = dp[2] = dp[3] = -0.0;
+            return;
+        }
+        double c[64];
+        christoffel(P, y, c);
+        contract<Alcubierre>(c, y + 4, dp);
+    }
+    GEON_D static int terminate(const geon_dmetric& P, const double* x) {
+        const double r2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+        if (!__builtin_isfinite(r2)) return DIVERGED;
+        return (r2 > P.esc2) ? ESCAPED : RUNNING;
+    }
+};
+
+// ---- integrator (integrator.rs:16-40) -------------------------------------
+// k_i = (p_i, dp_i); the combination ((k1 + 2k2) + 2k3) + k4 is accumulated
+// in that order.
+template <class MP>
+GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out) {
+    double acc[8], yt[8], dp[4];
+    const double hh = 0.5 * h;
+    MP::accel(P, y, dp);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        acc[i] = y[4 + i];
+        acc[4 + i] = dp[i];
+        yt[i] = y[i] + hh * y[4 + i];
+        yt[4 + i] = y[4 + i] + hh * dp[i];
+    }
+    MP::accel(P, yt, dp);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double kx = yt[4 + i];
+        acc[i] = acc[i] + 2.0 * kx;
+        acc[4 + i] = acc[4 + i] + 2.0 * dp[i];
+        yt[i] = y[i] + hh * kx;
+        yt[4 + i] = y[4 + i] + hh * dp[i];
+    }
+    MP::accel(P, yt, dp);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double kx = yt[4 + i];
+        acc[i] = acc[i] + 2.0 * kx;
+        acc[4 + i] = acc[4 + i] + 2.0 * dp[i];
+        yt[i] = y[i] + h * kx;
+        yt[4 + i] = y[4 + i] + h * dp[i];
+    }
+    MP::accel(P, yt, dp);
+    const double sixth = h / 6.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        acc[i] = acc[i] + yt[4 + i];
+        acc[4 + i] = acc[4 + i] + dp[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
+}
+
+// |g(p,p)| at the final position (geodesic.rs:75-77).
+template <class MP>
+GEON_D double null_residual(const geon_dmetric& P, const double* y) {
+    double g[16];
+    MP::g(P, y, g);
+    return fabs(dot4<MP>(g, y + 4, y + 4));
+}
+
+// asymptotic_direction (metric/mod.rs:91-108; alcubierre.rs:81-88;
+// morris_thorne.rs:54-77)
+template <class MP>
+GEON_D void direction(const double* y, double* d) {
+    const double* x = y;
+    const double* p = y + 4;
+    if constexpr (std::is_same<MP, Alcubierre>::value) {
+        const double vx = p[1], vy = p[2], vz = p[3];
+        const double n = fmax(sqrt(vx * vx + vy * vy + vz * vz), 1e-30);
+        d[0] = vx / n; d[1] = vy / n; d[2] = vz / n;
+    } else {
+        double r, sign;
+        if constexpr (std::is_same<MP, MorrisThorne>::value) {
+            r = fmax(fabs(x[1]), 1e-30);
+            sign = (x[1] < 0.0) ? -1.0 : 1.0;
+        } else {
+            r = x[1];
+            sign = 1.0;
+        }
+        double sin_t, cos_t, sin_p, cos_p;
+        sincos(x[2], &sin_t, &cos_t);
+        sincos(x[3], &sin_p, &cos_p);
+        const double pr = p[1], pt = p[2], pp = p[3];
+        const double vx = sin_t * cos_p * pr + r * cos_t * cos_p * pt - r * sin_t * sin_p * pp;
+        const double vy = sin_t * sin_p * pr + r * cos_t * sin_p * pt + r * sin_t * cos_p * pp;
+        const double vz = cos_t * pr - r * sin_t * pt;
+        const double n = fmax(sqrt(vx * vx + vy * vy + vz * vz), 1e-30);
+        d[0] = sign * vx / n; d[1] = vy / n; d[2] = vz / n;
+    }
+}
+
+// trace (geodesic.rs:62-117): terminate first, step, NaN guard keeps the
+// pre-step state, max_steps → MAXSTEPS.
+template <class MP>
+GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps, uint32_t* steps) {
+    for (uint32_t n = 0; n < max_steps; ++n) {
+        const int t = MP::terminate(P, y);
+        if (t != RUNNING) {
+            *steps = n;
+            return t;
+        }
+        double yn[8];
+        rk4_step<MP>(P, y, h, yn);
+        if (!finite8(yn)) {
+            *steps = n;
+            return DIVERGED;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) y[i] = yn[i];
+    }
+    *steps = max_steps;
+    return MAXSTEPS;
+}
+
+// camera.rs:113-160 — pixel (i, j) + (du, dv) → initial state.
+GEON_D void camera_ray(const geon_dframe& F, uint32_t i, uint32_t j, double du, double dv,
+                       double* y) {
+    const double u = (2.0 * ((double)i + du) / (double)F.W - 1.0) * F.w_im;
+    const double v = -(2.0 * ((double)j + dv) / (double)F.H - 1.0) * F.h_im;
+    const double dx = -1.0, dy = -v, dz = u;
+    const double norm = sqrt(dx * dx + dy * dy + dz * dz);
+    const double n1 = dx / norm, n2 = dy / norm, n3 = dz / norm;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu) {
+        y[mu] = F.obs[mu];
+        y[4 + mu] = F.tet[mu] + n1 * F.tet[4 + mu] + n2 * F.tet[8 + mu] + n3 * F.tet[12 + mu];
+    }
+}
+
+// ---- sky (sky.rs) -----------------------------------------------------------
+#define GEON_TAU 6.283185307179586
+#define GEON_PI 3.141592653589793
+
+GEON_D long long f2i64(double v) {
+    if (__builtin_isnan(v)) return 0;
+    if (v >= 9223372036854775807.0) return 9223372036854775807LL;
+    if (v <= -9223372036854775808.0) return (-9223372036854775807LL - 1);
+    return (long long)v;
+}
+GEON_D double rem_euclid(double a, double b) {
+    const double r = fmod(a, b);
+    return (r < 0.0) ? r + fabs(b) : r;
+}
+GEON_D long long rem_euclid_i(long long a, long long b) {
+    const long long r = a % b;
+    return (r < 0) ? r + b : r;
+}
+GEON_D double clampd(double x, double lo, double hi) {
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return x;
+}
+GEON_D unsigned long long sky_hash1(unsigned long long x) {
+    x *= 0x9E3779B97F4A7C15ull;
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return x;
+}
+GEON_D void sky_sample(const double* dir, bool show_stars, float* rgb) {
+    const double x = dir[0], y = dir[1], z = dir[2];
+    const double r = fmax(sqrt(x * x + y * y + z * z), 1e-30);
+    const double theta = acos(clampd(z / r, -1.0, 1.0));
+    const double phi = atan2(y, x);
+    const double phi_shift = rem_euclid(phi + 0.37, GEON_TAU);
+    const double theta_shift = clampd(theta + 0.21, 0.0, GEON_PI);
+    // sectors (sky.rs:72-104)
+    const double pal[6][3] = {{0.15, 0.08, 0.20}, {0.08, 0.15, 0.22}, {0.20, 0.15, 0.08},
+                              {0.10, 0.18, 0.10}, {0.22, 0.10, 0.10}, {0.10, 0.10, 0.22}};
+    const long long phi_mod = f2i64(floor(phi_shift / GEON_TAU * 6.0));
+    const long long band = f2i64(floor(theta_shift / GEON_PI * 3.0));
+    const int sec = (int)rem_euclid_i(phi_mod, 6);
+    const double band_mod = (rem_euclid_i(band, 2) == 0) ? 1.0 : 0.6;
+    const long long ic = f2i64(floor(phi_shift / GEON_TAU * 24.0));
+    const long long jc = f2i64(floor(theta_shift / GEON_PI * 12.0));
+    const double check = (rem_euclid_i(ic + jc, 2) == 0) ? 1.0 : 0.55;
+    float s[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s[c] = (float)(pal[sec][c] * band_mod * check);
+    // stars (sky.rs:43-68)
+    float st[3] = {0.0f, 0.0f, 0.0f};
+    if (show_stars) {
+        const long long i = f2i64(floor(phi_shift / GEON_TAU * 512.0));
+        const long long j = f2i64(floor(theta_shift / GEON_PI * 256.0));
+        const unsigned long long h =
+            sky_hash1(((unsigned long long)i * 73856093ull) ^ ((unsigned long long)j * 19349663ull));
+        const double prob = (double)(h & 0xFFFF) / 65535.0;
+        if (prob < 0.004) {
+            const double br = 0.4 + 0.6 * ((double)((h >> 16) & 0xFF) / 255.0);
+            const double temp = (double)((h >> 24) & 0xFF) / 255.0;
+            st[0] = (float)(br * (1.0 - 0.3 * temp));
+            st[1] = (float)(br * (1.0 - 0.1 * fabs(temp - 0.5)));
+            st[2] = (float)(br * (0.7 + 0.3 * temp));
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) rgb[c] = fminf(s[c] + st[c], 1.0f);
+}
+
+}  // namespace geon

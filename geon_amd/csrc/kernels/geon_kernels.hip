@@ -1,0 +1,380 @@
+// geon_kernels.hip — gfx950 kernels of the geodesic path, built as a code
+// object (hipcc --genco) and loaded by the host runtime with hipModuleLoad.
+//
+// Frame parameters (metric constants, camera tetrad, step schedule) live in
+// a ring of __constant__ slots written by the host before each launch; the
+// slot index is a kernel argument. Ray data is structure-of-arrays: plane k
+// of an n-ray array starts at base + k·n, so consecutive lanes touch
+// consecutive addresses.
+#include "geon_device.hpp"
+
+using namespace geon;
+
+extern "C" {
+__constant__ geon_dframe c_frames[GEON_CONST_SLOTS];
+}
+
+// ---- render: one lane per (pixel, sub-sample) ray -------------------------
+// ray r → pixel r / spa², sub-sample r % spa² (render.rs:89-92), so the
+// sub-rays of one pixel share a wave and their trajectories stay coherent.
+template <class MP>
+__device__ void render_rays(uint32_t slot, uint64_t n_rays, int32_t* __restrict__ cls,
+                            uint32_t* __restrict__ steps, double* __restrict__ resid,
+                            double* __restrict__ dir) {
+    const geon_dframe& F = c_frames[slot];
+    const uint64_t ray = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (ray >= n_rays) return;
+    const uint32_t spa = F.spa, spa2 = spa * spa;
+    const uint64_t pix = ray / spa2;
+    const uint32_t sub = (uint32_t)(ray - pix * spa2);
+    const uint32_t i = (uint32_t)(pix % F.W), j = (uint32_t)(pix / F.W);
+    const double du = ((double)(sub % spa) + 0.5) * F.inv_spa;
+    const double dv = ((double)(sub / spa) + 0.5) * F.inv_spa;
+    double y[8];
+    camera_ray(F, i, j, du, dv, y);
+    uint32_t n;
+    const int c = trace<MP>(F.m, y, F.step, F.max_steps, &n);
+    cls[ray] = c;
+    steps[ray] = n;
+    resid[ray] = null_residual<MP>(F.m, y);
+    double d[3] = {0.0, 0.0, 0.0};
+    if (c == ESCAPED) direction<MP>(y, d);
+    dir[ray] = d[0];
+    dir[n_rays + ray] = d[1];
+    dir[2 * n_rays + ray] = d[2];
+}
+
+// ---- trace from caller-supplied initial states ----------------------------
+template <class MP>
+__device__ void trace_states(uint32_t slot, uint64_t n, const double* __restrict__ y0,
+                             int32_t* __restrict__ cls, uint32_t* __restrict__ steps,
+                             double* __restrict__ yf, double* __restrict__ resid,
+                             double* __restrict__ dir) {
+    const geon_dframe& F = c_frames[slot];
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double y[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) y[c] = y0[c * n + k];
+    uint32_t st;
+    const int c = trace<MP>(F.m, y, F.step, F.max_steps, &st);
+    if (cls) cls[k] = c;
+    if (steps) steps[k] = st;
+    if (yf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) yf[q * n + k] = y[q];
+    }
+    if (resid) resid[k] = null_residual<MP>(F.m, y);
+    if (dir) {
+        double d[3] = {0.0, 0.0, 0.0};
+        if (c == ESCAPED) direction<MP>(y, d);
+        dir[k] = d[0];
+        dir[n + k] = d[1];
+        dir[2 * n + k] = d[2];
+    }
+}
+
+#define GEON_KERNELS(NAME, MP)                                                                  \
+    extern "C" __global__ __launch_bounds__(256) void geon_render_rays_##NAME(                  \
+        uint32_t slot, uint64_t n_rays, int32_t* cls, uint32_t* steps, double* resid,          \
+        double* dir) {                                                                          \
+        render_rays<MP>(slot, n_rays, cls, steps, resid, dir);                                  \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256) void geon_trace_states_##NAME(                 \
+        uint32_t slot, uint64_t n, const double* y0, int32_t* cls, uint32_t* steps,            \
+        double* yf, double* resid, double* dir) {                                               \
+        trace_states<MP>(slot, n, y0, cls, steps, yf, resid, dir);                              \
+    }
+
+GEON_KERNELS(mink, Minkowski)
+GEON_KERNELS(schw, Schwarzschild)
+GEON_KERNELS(alc, Alcubierre)
+GEON_KERNELS(mt, MorrisThorne)
+
+// ---- per-pixel reduction, sky colour and frame statistics ------------------
+// render.rs:96-163: colour per sub-ray, f32 average, majority class with
+// ties captured > diverged > escaped, integer per-pixel step average, max
+// finite residual; statistics summed over pixels.
+extern "C" __global__ __launch_bounds__(256) void geon_render_finalize(
+    uint32_t slot, uint64_t n_pix, const int32_t* __restrict__ cls, const uint32_t* __restrict__ steps,
+    const double* __restrict__ resid, const double* __restrict__ dir, float* __restrict__ rgb,
+    int32_t* __restrict__ cls_pix, double* __restrict__ dir_pix, geon_dstats* __restrict__ stats) {
+    const geon_dframe& F = c_frames[slot];
+    const uint64_t pix = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t spa2 = F.spa * F.spa;
+    const uint64_t n_rays = n_pix * spa2;
+    unsigned long long l_cap = 0, l_esc = 0, l_div = 0, l_max = 0, l_rs = 0, l_sp = 0;
+    unsigned long long l_resid = 0;
+    if (pix < n_pix) {
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+        uint32_t nc = 0, ne = 0, nd = 0, nmax = 0;
+        unsigned long long total = 0;
+        double mres = 0.0;
+        double fd[3] = {0.0, 0.0, 0.0};
+        bool have = false;
+        for (uint32_t s = 0; s < spa2; ++s) {
+            const uint64_t r = pix * spa2 + s;
+            const int c = cls[r];
+            float col[3];
+            if (c == CAPTURED) {
+                ++nc;
+                col[0] = F.captured_rgb[0]; col[1] = F.captured_rgb[1]; col[2] = F.captured_rgb[2];
+            } else if (c == ESCAPED) {
+                ++ne;
+                const double d[3] = {dir[r], dir[n_rays + r], dir[2 * n_rays + r]};
+                if (!have) { fd[0] = d[0]; fd[1] = d[1]; fd[2] = d[2]; have = true; }
+                sky_sample(d, F.show_stars != 0, col);
+            } else {
+                ++nd;
+                if (c == MAXSTEPS) ++nmax;
+                col[0] = F.diverged_rgb[0]; col[1] = F.diverged_rgb[1]; col[2] = F.diverged_rgb[2];
+            }
+            acc[0] += col[0]; acc[1] += col[1]; acc[2] += col[2];
+            total += steps[r];
+            const double rr = resid[r];
+            if (__builtin_isfinite(rr) && rr > mres) mres = rr;
+        }
+        const float nf = (float)spa2;
+        if (rgb) {
+            rgb[3 * pix] = acc[0] / nf;
+            rgb[3 * pix + 1] = acc[1] / nf;
+            rgb[3 * pix + 2] = acc[2] / nf;
+        }
+        int term;
+        if (nd > 0 && nd >= nc && nd >= ne) term = (nmax == nd) ? MAXSTEPS : DIVERGED;
+        else if (nc >= ne) term = CAPTURED;
+        else term = ESCAPED;
+        if (cls_pix) cls_pix[pix] = term;
+        if (dir_pix) {
+            dir_pix[pix] = fd[0];
+            dir_pix[n_pix + pix] = fd[1];
+            dir_pix[2 * n_pix + pix] = fd[2];
+        }
+        l_cap = term == CAPTURED;
+        l_esc = term == ESCAPED;
+        l_div = (term == DIVERGED || term == MAXSTEPS);
+        l_max = nmax;
+        l_rs = total;
+        l_sp = total / spa2;
+        l_resid = (unsigned long long)__double_as_longlong(mres);
+    }
+    // block reduction, then one atomic per counter per block
+    __shared__ unsigned long long red[7][256];
+    const int t = threadIdx.x;
+    red[0][t] = l_cap; red[1][t] = l_esc; red[2][t] = l_div; red[3][t] = l_max;
+    red[4][t] = l_rs; red[5][t] = l_sp; red[6][t] = l_resid;
+    __syncthreads();
+    for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+        if (t < w) {
+#pragma unroll
+            for (int q = 0; q < 6; ++q) red[q][t] += red[q][t + w];
+            red[6][t] = red[6][t] > red[6][t + w] ? red[6][t] : red[6][t + w];
+        }
+        __syncthreads();
+    }
+    if (t == 0 && stats) {
+        atomicAdd(&stats->captured, red[0][0]);
+        atomicAdd(&stats->escaped, red[1][0]);
+        atomicAdd(&stats->diverged, red[2][0]);
+        atomicAdd(&stats->max_steps_hit, red[3][0]);
+        atomicAdd(&stats->ray_steps, red[4][0]);
+        atomicAdd(&stats->steps_pix_sum, red[5][0]);
+        atomicMax(&stats->max_resid_bits, red[6][0]);
+    }
+}
+
+// Zero a statistics block (used instead of a memset node so the sequence
+// stays valid inside captured graphs).
+extern "C" __global__ void geon_stats_clear(geon_dstats* stats) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        stats->captured = stats->escaped = stats->diverged = 0;
+        stats->max_resid_bits = stats->steps_pix_sum = stats->max_steps_hit = 0;
+        stats->ray_steps = stats->fp64_rays = 0;
+    }
+}
+
+// ---- curvature (curvature.rs:25-141) ----------------------------------------
+// One lane per point. ∂_d Γ by central differences of Γ (times 1/(2h),
+// curvature.rs:34-44) only along coordinates Γ depends on; the other
+// directions difference identical evaluations and are exactly zero.
+// Riemann R^ρ_σμν = ((∂_μΓ^ρ_νσ − ∂_νΓ^ρ_μσ) + Σ_λ Γ^ρ_μλΓ^λ_νσ) − Σ_λ Γ^ρ_νλΓ^λ_μσ
+// (curvature.rs:48-70); Ricci R_σν = Σ_α R^α_σαν; R = Σ g^μν R_μν;
+// G = R_μν − ½ g_μν R; T = G·(1/(8π)). Structural zeros are skipped with
+// the masks of the metric policy, which leaves every sum unchanged.
+template <class MP>
+__device__ void curvature_point(uint32_t slot, uint64_t n, const double* __restrict__ xs,
+                                int32_t what, double* __restrict__ out,
+                                double* __restrict__ scratch, double factor) {
+    const geon_dframe& F = c_frames[slot];
+    const geon_dmetric& P = F.m;
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const double x[4] = {xs[k], xs[n + k], xs[2 * n + k], xs[3 * n + k]};
+    double gam[64];
+    MP::christoffel(P, x, gam);
+    auto nz = [](int r, int a, int b) constexpr { return MP::gam_nz(r, a, b); };
+    (void)nz;
+
+    if (what == 0) {  // full Riemann: ∂Γ through a per-point scratch area
+        sfor<4>([&](auto D) {
+            constexpr int d = CI(D);
+            if constexpr (MP::dep(d)) {
+                double xp[4] = {x[0], x[1], x[2], x[3]}, xm[4] = {x[0], x[1], x[2], x[3]};
+                xp[d] += P.fd_h;
+                xm[d] -= P.fd_h;
+                double gp[64], gm[64];
+                MP::christoffel(P, xp, gp);
+                MP::christoffel(P, xm, gm);
+                sfor<64>([&](auto I) {
+                    constexpr int i = CI(I);
+                    if constexpr (MP::gam_nz(i / 16, (i / 4) % 4, i % 4))
+                        scratch[(uint64_t)(d * 64 + i) * n + k] = (gp[i] - gm[i]) * P.fd_inv_2h;
+                });
+            }
+        });
+        for (int rho = 0; rho < 4; ++rho)
+            for (int sg = 0; sg < 4; ++sg)
+                for (int mu = 0; mu < 4; ++mu)
+                    for (int nu = 0; nu < 4; ++nu) {
+                        double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+                        if (MP::dep(mu) && MP::gam_nz(rho, nu, sg))
+                            t1 = scratch[(uint64_t)(mu * 64 + rho * 16 + nu * 4 + sg) * n + k];
+                        if (MP::dep(nu) && MP::gam_nz(rho, mu, sg))
+                            t2 = scratch[(uint64_t)(nu * 64 + rho * 16 + mu * 4 + sg) * n + k];
+                        for (int l = 0; l < 4; ++l) {
+                            if (MP::gam_nz(rho, mu, l) && MP::gam_nz(l, nu, sg))
+                                t3 += gam[rho * 16 + mu * 4 + l] * gam[l * 16 + nu * 4 + sg];
+                            if (MP::gam_nz(rho, nu, l) && MP::gam_nz(l, mu, sg))
+                                t4 += gam[rho * 16 + nu * 4 + l] * gam[l * 16 + mu * 4 + sg];
+                        }
+                        out[k * 256 + rho * 64 + sg * 16 + mu * 4 + nu] = t1 - t2 + t3 - t4;
+                    }
+        return;
+    }
+
+    // Ricci needs D1[d][ν][σ] = ∂_d Γ^d_νσ and D2[d][α][σ] = ∂_d Γ^α_ασ only.
+    double D1[64], D2[64];
+    sfor<4>([&](auto D) {
+        constexpr int d = CI(D);
+        if constexpr (MP::dep(d)) {
+            double xp[4] = {x[0], x[1], x[2], x[3]}, xm[4] = {x[0], x[1], x[2], x[3]};
+            xp[d] += P.fd_h;
+            xm[d] -= P.fd_h;
+            double gp[64], gm[64];
+            MP::christoffel(P, xp, gp);
+            MP::christoffel(P, xm, gm);
+            sfor<4>([&](auto B) {
+                sfor<4>([&](auto C) {
+                    constexpr int b = CI(B), c = CI(C);
+                    if constexpr (MP::gam_nz(d, b, c))
+                        D1[d * 16 + b * 4 + c] =
+                            (gp[d * 16 + b * 4 + c] - gm[d * 16 + b * 4 + c]) * P.fd_inv_2h;
+                    if constexpr (MP::gam_nz(b, b, c))
+                        D2[d * 16 + b * 4 + c] =
+                            (gp[b * 16 + b * 4 + c] - gm[b * 16 + b * 4 + c]) * P.fd_inv_2h;
+                });
+            });
+        }
+    });
+    double ric[16];
+    sfor<4>([&](auto Sg) {
+        sfor<4>([&](auto Nu) {
+            constexpr int sg = CI(Sg), nu = CI(Nu);
+            double s = 0.0;
+            sfor<4>([&](auto A) {
+                constexpr int a = CI(A);
+                double t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+                if constexpr (MP::dep(a) && MP::gam_nz(a, nu, sg)) t1 = D1[a * 16 + nu * 4 + sg];
+                if constexpr (MP::dep(nu) && MP::gam_nz(a, a, sg)) t2 = D2[nu * 16 + a * 4 + sg];
+                sfor<4>([&](auto L) {
+                    constexpr int l = CI(L);
+                    if constexpr (MP::gam_nz(a, a, l) && MP::gam_nz(l, nu, sg))
+                        t3 += gam[a * 16 + a * 4 + l] * gam[l * 16 + nu * 4 + sg];
+                    if constexpr (MP::gam_nz(a, nu, l) && MP::gam_nz(l, a, sg))
+                        t4 += gam[a * 16 + nu * 4 + l] * gam[l * 16 + a * 4 + sg];
+                });
+                s += t1 - t2 + t3 - t4;
+            });
+            ric[sg * 4 + nu] = s;
+        });
+    });
+    if (what == 1) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) out[k * 16 + i] = ric[i];
+        return;
+    }
+    double gi[16];
+    MP::ginv_at(P, x, gi);
+    double R = 0.0;
+    sfor<16>([&](auto I) {
+        constexpr int i = CI(I);
+        if constexpr (MP::ginv_nz(i / 4, i % 4)) R += gi[i] * ric[i];
+    });
+    if (what == 4) {
+        out[k] = R;
+        return;
+    }
+    double g[16];
+    MP::g(P, x, g);
+    sfor<16>([&](auto I) {
+        constexpr int i = CI(I);
+        double e = ric[i];
+        if constexpr (MP::g_nz(i / 4, i % 4)) e = ric[i] - 0.5 * g[i] * R;
+        out[k * 16 + i] = (what == 3) ? e * factor : e;
+    });
+}
+
+#define GEON_CURV_KERNEL(NAME, MP)                                                              \
+    extern "C" __global__ __launch_bounds__(256) void geon_curvature_##NAME(                    \
+        uint32_t slot, uint64_t n, const double* x, int32_t what, double* out, double* scratch,\
+        double factor) {                                                                        \
+        curvature_point<MP>(slot, n, x, what, out, scratch, factor);                            \
+    }
+
+GEON_CURV_KERNEL(mink, Minkowski)
+GEON_CURV_KERNEL(schw, Schwarzschild)
+GEON_CURV_KERNEL(alc, Alcubierre)
+GEON_CURV_KERNEL(mt, MorrisThorne)
+
+// ---- shadow bisection (validate_shadow.rs:20-95) ----------------------------
+// One lane per problem k: Schwarzschild::new(mass[k]) (pad 1.001, escape
+// 1000 M), equatorial launch at angle α from the inward radial direction,
+// trace(step, max_steps) capture as the predicate, `iters` halvings of
+// [0, π/2 − 1e-3].
+__device__ void equatorial_ray(double m, double r_obs, double alpha, double* y) {
+    const double f = 1.0 - 2.0 * m / r_obs;
+    double sin_a, cos_a;
+    sincos(alpha, &sin_a, &cos_a);
+    y[0] = 0.0; y[1] = r_obs; y[2] = 1.5707963267948966; y[3] = 0.0;
+    y[4] = 1.0 / sqrt(f);
+    y[5] = -cos_a * sqrt(f);
+    y[6] = 0.0;
+    y[7] = sin_a / r_obs;
+}
+
+extern "C" __global__ __launch_bounds__(64) void geon_bisect_serial(
+    uint64_t n, const double* __restrict__ mass, const double* __restrict__ r_obs, double step,
+    uint32_t max_steps, uint32_t iters, double* __restrict__ alpha_c, double* __restrict__ b_c) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    geon_dmetric P = {};
+    P.kind = 1;
+    P.M = mass[k];
+    P.cap_r = 1.001 * 2.0 * P.M;
+    P.esc_r = 1000.0 * P.M;
+    const double r = r_obs[k];
+    double lo = 0.0, hi = 1.5707963267948966 - 1e-3;
+    for (uint32_t it = 0; it < iters; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        double y[8];
+        equatorial_ray(P.M, r, mid, y);
+        uint32_t s;
+        const int c = trace<Schwarzschild>(P, y, step, max_steps, &s);
+        if (c == CAPTURED) lo = mid;
+        else hi = mid;
+    }
+    const double a = 0.5 * (lo + hi);
+    const double f = 1.0 - 2.0 * P.M / r;
+    alpha_c[k] = a;
+    b_c[k] = r * sin(a) / sqrt(f);
+}

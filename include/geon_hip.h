@@ -133,6 +133,8 @@ typedef struct geon_render_stats {
 typedef struct geon_ctx geon_ctx_t;
 
 const char* geon_strerror(int status);
+/* Detail of the last failure on the calling thread (empty if none). */
+const char* geon_last_error(void);
 int  geon_version(void);
 
 int  geon_metric_default(int32_t kind, geon_metric_t* out);

@@ -55,6 +55,9 @@ class TraceResult(C.Structure):
 
 _lib = None
 
+# worker threads for the batched oracle calls (the GPU box grants 16 cores)
+THREADS = max(1, min(16, os.cpu_count() or 1))
+
 
 def build() -> None:
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -194,7 +197,7 @@ def trace(m, y0, step, max_steps):
     return r.cls, r.steps, np.array(r.y[:]), r.null_residual
 
 
-def trace_rays(m, y0_aos, step, max_steps, nthreads=os.cpu_count() or 1):
+def trace_rays(m, y0_aos, step, max_steps, nthreads=THREADS):
     y0 = np.ascontiguousarray(y0_aos, dtype=np.float64)
     n = y0.shape[0]
     cls = np.zeros(n, np.int32)
@@ -247,7 +250,7 @@ def tonemap_u8(px):
     return out.reshape(px.shape)
 
 
-def render(m, cam, cfg, nthreads=os.cpu_count() or 1):
+def render(m, cam, cfg, nthreads=THREADS):
     n = cam.width * cam.height
     rgb = np.zeros((n, 3), np.float32)
     cls = np.zeros(n, np.int32)
@@ -293,7 +296,7 @@ def nec_contraction(m, x, k):
     return lib().or_nec_contraction(C.byref(m), _dp(_x4(x)), _dp(_x4(k)))
 
 
-def curvature(m, x_aos, what, nthreads=os.cpu_count() or 1):
+def curvature(m, x_aos, what, nthreads=THREADS):
     x = np.ascontiguousarray(x_aos, dtype=np.float64)
     n = x.shape[0]
     width = {CURV_RIEMANN: 256, CURV_SCALAR: 1}.get(what, 16)
