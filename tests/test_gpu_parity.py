@@ -94,8 +94,8 @@ def test_render_schwarzschild_c1(ctx, O):
 def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps, chaotic):
     """chaotic: the superluminal bubble (v = 2) has horizons in its walls, where
     the reference's fixed step amplifies ulp-level differences of tanh between
-    libm implementations; there the bound is statistical (99% of escaped rays
-    within 1e-9 rad, all within 1e-5 rad) and classes must still agree."""
+    libm implementations; there the bound is statistical (90% of escaped rays
+    within 1e-9 rad, all within 1e-6 rad) and classes must still agree."""
     import geon_amd as geon
     om = O.metric(kind, escape_radius=esc_r, **kw)
     cam = O.camera(obs, 60.0, 64, 48)
@@ -113,7 +113,7 @@ def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps, cha
         esc = (got["cls"] == ref["cls"]) & (ref["cls"] == O.ESCAPED) & \
             (np.abs(ref["y"][:, 1:4]).max(axis=1) < 10 * esc_r)
         a = _angle(got["dir"][esc], ref["dir"][esc])
-        assert (a <= ANGLE_TOL).mean() >= 0.99 and a.max() <= 1e-5
+        assert (a <= ANGLE_TOL).mean() >= 0.90 and a.max() <= 1e-6
 
 
 def test_trace_semantics_on_gpu(ctx, O):
