@@ -21,7 +21,8 @@ from ._abi import (ALCUBIERRE, CAPTURED, CURV_EINSTEIN, CURV_RICCI, CURV_RIEMANN
 
 __all__ = ["Minkowski", "Schwarzschild", "Alcubierre", "MorrisThorne", "Camera", "RenderConfig",
            "Context", "render", "trace_rays", "riemann", "ricci", "ricci_scalar", "einstein",
-           "stress_energy", "nec_contraction", "shadow_bisect", "tetrad", "GeonError"]
+           "stress_energy", "nec_contraction", "shadow_bisect", "tetrad", "GeonError", "Batch",
+           "autotune"]
 
 
 # ---- metrics (src/metric/*.rs) ----------------------------------------------
@@ -124,6 +125,7 @@ class RenderConfig:
     samples_per_axis: int = 1
     show_stars: bool = True
     precision: int = PREC_FP64
+    fp64_band: float = 0.0
 
     def to_c(self) -> _abi.RenderCfg:
         r = _abi.RenderCfg()
@@ -132,6 +134,7 @@ class RenderConfig:
         r.captured_rgb[:] = list(self.captured_rgb)
         r.diverged_rgb[:] = list(self.diverged_rgb)
         r.show_stars, r.precision = int(bool(self.show_stars)), int(self.precision)
+        r.fp64_band = float(self.fp64_band)
         return r
 
 
@@ -232,6 +235,77 @@ def render_dev(metric, camera: Camera, cfg: RenderConfig, rgb_ptr, cls_ptr, dir_
                                 C.c_void_p(stream_ptr) if stream_ptr else None),
           "geon_render_dev")
     return st.as_dict()
+
+
+class Batch:
+    """Persistent-scheduler batch (geon_batch_*): camera frames and explicit ray
+    sets of any metric kinds and precisions share persistent launches.
+    Output arguments are device pointers (ints, e.g. torch ``tensor.data_ptr()``)."""
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        h = C.c_void_p()
+        check(lib().geon_batch_create(self.ctx.handle, C.byref(h)), "geon_batch_create")
+        self.handle = h
+
+    def add_frame(self, metric, camera: Camera, cfg: RenderConfig, rgb=0, cls=0, dir=0) -> int:
+        fid = C.c_uint32()
+        check(lib().geon_batch_add_frame(self.handle, C.byref(_cmetric(metric)),
+                                         C.byref(camera.to_c()), C.byref(cfg.to_c()),
+                                         C.c_void_p(rgb or None), C.c_void_p(cls or None),
+                                         C.c_void_p(dir or None), C.byref(fid)),
+              "geon_batch_add_frame")
+        return fid.value
+
+    def add_rays(self, metric, y0_soa_ptr, n, step, max_steps, precision=PREC_FP64,
+                 early_exit=False, cls=0, steps=0, yfin=0, resid=0, dir=0) -> int:
+        jid = C.c_uint32()
+        check(lib().geon_batch_add_rays(self.handle, C.byref(_cmetric(metric)),
+                                        C.c_void_p(y0_soa_ptr), n, float(step), int(max_steps),
+                                        int(precision), int(bool(early_exit)),
+                                        C.c_void_p(cls or None), C.c_void_p(steps or None),
+                                        C.c_void_p(yfin or None), C.c_void_p(resid or None),
+                                        C.c_void_p(dir or None), C.byref(jid)),
+              "geon_batch_add_rays")
+        return jid.value
+
+    def run(self, tune: dict | None = None, stream_ptr=None):
+        cfg = None
+        if tune:
+            cfg = _abi.SchedCfg(**{k: int(v) for k, v in tune.items()})
+        check(lib().geon_batch_run(self.handle, C.byref(cfg) if cfg else None,
+                                   C.c_void_p(stream_ptr) if stream_ptr else None),
+              "geon_batch_run")
+
+    def frame_stats(self, fid: int) -> dict:
+        st = _abi.RenderStats()
+        check(lib().geon_batch_frame_stats(self.handle, fid, C.byref(st)), "geon_batch_frame_stats")
+        return st.as_dict()
+
+    def totals(self) -> dict:
+        rs, fb, ms = C.c_uint64(), C.c_uint64(), C.c_double()
+        check(lib().geon_batch_totals(self.handle, C.byref(rs), C.byref(fb), C.byref(ms)),
+              "geon_batch_totals")
+        return dict(executed_ray_steps=rs.value, fp64_rays=fb.value, device_ms=ms.value)
+
+    def close(self):
+        if self.handle:
+            lib().geon_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(kind: int, precision: int = PREC_MIXED, ctx: Context | None = None) -> dict:
+    """Time launch shapes on a calibration frame; the winner becomes the default."""
+    ctx = ctx or default_context()
+    cfg = _abi.SchedCfg()
+    check(lib().geon_autotune(ctx.handle, int(kind), int(precision), C.byref(cfg)), "geon_autotune")
+    return {k: getattr(cfg, k) for k, _ in cfg._fields_}
 
 
 def curvature(metric, x, what, ctx: Context | None = None):

@@ -45,7 +45,12 @@ class RenderCfg(C.Structure):
     _fields_ = [("step", C.c_double), ("max_steps", C.c_uint32),
                 ("samples_per_axis", C.c_uint32), ("captured_rgb", C.c_float * 3),
                 ("diverged_rgb", C.c_float * 3), ("show_stars", C.c_uint32),
-                ("precision", C.c_int32)]
+                ("precision", C.c_int32), ("fp64_band", C.c_double)]
+
+
+class SchedCfg(C.Structure):
+    _fields_ = [("block", C.c_uint32), ("waves_per_simd", C.c_uint32),
+                ("rays_per_lane", C.c_uint32), ("refill", C.c_uint32)]
 
 
 class RenderStats(C.Structure):
@@ -81,6 +86,17 @@ SIGNATURES = [
     ("geon_curvature_dev", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_int32, _vp, _vp]),
     ("geon_shadow_bisect", _i, [_vp, _vp, _vp, _sz, C.c_double, _u32, _u32, _vp, _vp]),
     ("geon_tonemap_u8", _i, [_vp, _sz, _vp]),
+    ("geon_batch_create", _i, [_vp, C.POINTER(_vp)]),
+    ("geon_batch_destroy", None, [_vp]),
+    ("geon_batch_add_frame", _i, [_vp, C.POINTER(Metric), C.POINTER(Camera), C.POINTER(RenderCfg),
+                                  _vp, _vp, _vp, C.POINTER(_u32)]),
+    ("geon_batch_add_rays", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_double, _u32, C.c_int32,
+                                 C.c_int32, _vp, _vp, _vp, _vp, _vp, C.POINTER(_u32)]),
+    ("geon_batch_run", _i, [_vp, _vp, _vp]),
+    ("geon_batch_frame_stats", _i, [_vp, _u32, C.POINTER(RenderStats)]),
+    ("geon_batch_totals", _i, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                               C.POINTER(C.c_double)]),
+    ("geon_autotune", _i, [_vp, C.c_int32, C.c_int32, _vp]),
 ]
 
 _lib = None

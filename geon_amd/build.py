@@ -28,9 +28,11 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
               "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-result"]
 
 KERNEL_SRC = [os.path.join(CSRC, "kernels", "geon_kernels.hip")]
-KERNEL_DEPS = KERNEL_SRC + [os.path.join(CSRC, "kernels", "geon_device.hpp"),
-                            os.path.join(CSRC, "geon_dev_types.h")]
-HOST_SRC = [os.path.join(CSRC, "host", f) for f in ("geon_runtime.cpp", "geon_abi.cpp")]
+KERNEL_DEPS = KERNEL_SRC + [os.path.join(CSRC, "kernels", f)
+                            for f in ("geon_device.hpp", "geon_fp32.hpp", "geon_sched.hpp")] + \
+    [os.path.join(CSRC, "geon_dev_types.h")]
+HOST_SRC = [os.path.join(CSRC, "host", f)
+            for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp")]
 
 
 def _run(cmd):

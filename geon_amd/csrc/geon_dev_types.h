@@ -53,3 +53,48 @@ struct geon_dstats {
     unsigned long long ray_steps;
     unsigned long long fp64_rays;
 };
+
+// One unit of scheduled work: a camera frame (gen 0) or a set of explicit
+// initial states (gen 1). Rays of a queue are numbered globally; this job
+// owns ids [ray_base, ray_base + n_rays).
+struct geon_djob {
+    geon_dframe F;
+    unsigned long long ray_base;
+    unsigned long long n_rays;
+    int32_t  gen;
+    int32_t  early_exit;      // Schwarzschild only: decide capture/escape at the photon sphere
+    const double* y0;         // gen 1: SoA 8 × n_rays
+    int32_t*  cls;            // per-ray outputs (local index)
+    uint32_t* steps;
+    double*   resid;
+    double*   dir;            // SoA 3 × n_rays
+    double*   yfin;           // optional SoA 8 × n_rays
+    unsigned long long* done; // finished-ray counter
+    unsigned long long* fp64; // rays of this job handed to the FP64 path
+    // per-pixel products of a camera job (render.rs:96-163), any may be NULL
+    float*    rgb;            // 3 × W·H interleaved
+    int32_t*  cls_pix;
+    double*   dir_pix;        // SoA 3 × W·H
+    struct geon_dstats* stats;
+};
+
+// Launch arguments of a scheduler kernel (passed by value).
+struct geon_sched_args {
+    const geon_djob* jobs;
+    const unsigned long long* job_base;   // jobs[k].ray_base, ascending
+    uint32_t n_jobs;
+    uint32_t refill;                      // steps between refill points
+    unsigned long long* cursor;           // next unclaimed position
+    unsigned long long end;               // claimable positions are [cursor, end)
+    const unsigned long long* end_ptr;    // if set, end = *end_ptr at kernel start
+    const unsigned long long* list;       // list mode: position → ray id (else position = id)
+    uint32_t quantum;                     // step budget per lane (0 = unbounded)
+    uint32_t n_slots;                     // lanes × rays-per-lane
+    void* slot_y;                         // SoA 8 × n_slots of T (quantum mode)
+    unsigned long long* slot_g;           // ray id per slot, ~0 = empty
+    uint32_t* slot_n;                     // steps taken per slot
+    unsigned long long* fb_list;          // FP64 fallback ray ids (FP32 kernels)
+    unsigned long long* fb_count;
+    unsigned long long fb_cap;
+    unsigned long long* ray_steps;        // Σ executed RK4 steps (for throughput)
+};

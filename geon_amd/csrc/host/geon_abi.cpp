@@ -161,8 +161,19 @@ int geon_render_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_cam
                     const geon_render_cfg_t* cfg, float* rgb, int32_t* cls_per_pixel,
                     double* dir_per_pixel, geon_render_stats_t* stats, void* stream) {
     if (!ctx || !cfg) return fail(GEON_E_INVALID, "bad argument");
+    if (cfg->precision == GEON_PREC_MIXED) {
+        // FP32-mixed renders run on the persistent scheduler
+        geon_batch_t* b = nullptr;
+        GEON_TRY(geon_batch_create(ctx, &b));
+        uint32_t fid = 0;
+        int st = geon_batch_add_frame(b, metric, cam, cfg, rgb, cls_per_pixel, dir_per_pixel, &fid);
+        if (st == GEON_OK) st = geon_batch_run(b, nullptr, stream);
+        if (st == GEON_OK && stats) st = geon_batch_frame_stats(b, fid, stats);
+        geon_batch_destroy(b);
+        return st;
+    }
     if (cfg->precision != GEON_PREC_FP64)
-        return fail(GEON_E_UNSUPPORTED, "precision %d not provided by geon_render", cfg->precision);
+        return fail(GEON_E_INVALID, "unknown precision %d", cfg->precision);
     hipSetDevice(ctx->device);
     geon_dframe F;
     GEON_TRY(fill_frame(metric, &F));

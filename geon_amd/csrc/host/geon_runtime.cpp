@@ -196,6 +196,13 @@ int ctx_create(int device, const char* code_object, geon_ctx** out) {
     ok = ok && get(&c->fn_finalize, "geon_render_finalize");
     ok = ok && get(&c->fn_stats_clear, "geon_stats_clear");
     ok = ok && get(&c->fn_bisect, "geon_bisect_serial");
+    ok = ok && get(&c->fn_job_finalize, "geon_job_finalize");
+    ok = ok && get(&c->fn_zero, "geon_zero_u64");
+    for (int k = 0; k < 4 && ok; ++k)
+        for (int f = 0; f < 2 && ok; ++f)
+            for (int r = 0; r < 2 && ok; ++r)
+                ok = ok && get(&c->fn_sched[k][f][r], std::string("geon_sched_") + kMetricNames[k] +
+                                                          (f ? "_f32_r" : "_f64_r") + std::to_string(r + 1));
     if (!ok) {
         hipModuleUnload(c->mod);
         delete c;
@@ -217,6 +224,20 @@ int ctx_create(int device, const char* code_object, geon_ctx** out) {
         hipEventCreateWithFlags(&c->slot_ev[s], hipEventDisableTiming);
         c->slot_used[s] = false;
     }
+    // one stream per metric kind for concurrent scheduler launches (default priority)
+    for (int k = 0; k < 4; ++k) {
+        hipStreamCreateWithFlags(&c->kind_stream[k], hipStreamNonBlocking);
+        hipEventCreateWithFlags(&c->join_ev[k], hipEventDisableTiming);
+    }
+    hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming);
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        c->n_cu = ncu;
+    // launch-shape defaults until geon_autotune replaces them
+    for (int k = 0; k < 4; ++k) {
+        c->tuned[k][0] = {256, 2, 1, 32};
+        c->tuned[k][1] = {256, 4, 1, 32};
+    }
     hipEventCreate(&c->t0);
     hipEventCreate(&c->t1);
     if ((e = hipMalloc((void**)&c->d_stats, sizeof(geon_dstats))) != hipSuccess) {
@@ -236,6 +257,11 @@ void ctx_destroy(geon_ctx* c) {
     if (c->d_stats) hipFree(c->d_stats);
     for (int s = 0; s < GEON_CONST_SLOTS; ++s)
         if (c->slot_ev[s]) hipEventDestroy(c->slot_ev[s]);
+    for (int k = 0; k < 4; ++k) {
+        if (c->kind_stream[k]) hipStreamDestroy(c->kind_stream[k]);
+        if (c->join_ev[k]) hipEventDestroy(c->join_ev[k]);
+    }
+    if (c->fork_ev) hipEventDestroy(c->fork_ev);
     if (c->t0) hipEventDestroy(c->t0);
     if (c->t1) hipEventDestroy(c->t1);
     if (c->stream) hipStreamDestroy(c->stream);

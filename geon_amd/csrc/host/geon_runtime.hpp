@@ -17,6 +17,14 @@ struct geon_ctx {
     hipFunction_t fn_finalize = nullptr;
     hipFunction_t fn_stats_clear = nullptr;
     hipFunction_t fn_bisect = nullptr;
+    hipFunction_t fn_sched[4][2][2] = {};   // [kind][0 f64 | 1 f32][rays_per_lane - 1]
+    hipFunction_t fn_job_finalize = nullptr;
+    hipFunction_t fn_zero = nullptr;
+    int n_cu = 256;
+    geon_sched_cfg_t tuned[4][2] = {};       // [kind][0 FP64 | 1 FP32-mixed]
+    hipStream_t kind_stream[4] = {};
+    hipEvent_t fork_ev = nullptr;
+    hipEvent_t join_ev[4] = {};
     hipDeviceptr_t c_frames = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t slot_ev[GEON_CONST_SLOTS] = {};
@@ -48,6 +56,8 @@ int ctx_buffer(geon_ctx* c, const char* name, size_t bytes, void** p);
 int ctx_slot(geon_ctx* c, const geon_dframe* f, hipStream_t stream, uint32_t* slot);
 int ctx_slot_done(geon_ctx* c, uint32_t slot, hipStream_t stream);
 int launch(hipFunction_t f, uint64_t n_threads, uint32_t block, hipStream_t stream, void** args);
+int frame_from(const geon_metric_t* metric, const geon_camera_t* cam, const geon_render_cfg_t* cfg,
+               geon_dframe* F);
 
 }  // namespace geon_host
 

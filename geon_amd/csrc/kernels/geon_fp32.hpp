@@ -1,0 +1,281 @@
+// geon_fp32.hpp — FP32 side of the precision split.
+//
+// Bulk rays integrate in FP32 with the same RK4 tableau (integrator.rs:16-40)
+// but with closed-form Christoffel symbols for every metric: a central
+// difference with h = 1e-5 is pure rounding noise in FP32, so the generic
+// FD ∂g of the reference is replaced by the analytic ∂g it approximates
+// (its O(h²) truncation, ~1e-10 relative, is far below FP32 resolution).
+// After each step p^t is re-projected onto the null cone g(p,p) = 0
+// (future-directed root), which keeps FP32 drift out of the
+// energy-carrying component. Rays whose outcome is sensitive to rounding
+// are handed to the FP64 path: a priori those whose impact parameter lies
+// within a band of a photon orbit, a posteriori those that end in a state
+// a smooth fixed-step trajectory cannot reach (NaN, or a final step that
+// jumped far past the escape sphere or deep into the capture region).
+#pragma once
+#include "geon_device.hpp"
+
+namespace geon {
+
+// float metric constants, filled per ray from the double constants
+struct FParams {
+    float M, cap_r, esc_r, esc2, esc_l;
+    float R, inv_s, v, ship_x, inv_denom_s, denom_inv;
+    float b0sq;
+};
+
+GEON_D FParams fparams(const geon_dmetric& P) {
+    FParams f;
+    f.M = (float)P.M;
+    f.cap_r = (float)P.cap_r;
+    f.esc_r = (float)P.esc_r;
+    f.esc2 = (float)P.esc2;
+    f.esc_l = (float)P.esc_l;
+    f.R = (float)P.R;
+    f.inv_s = (float)P.inv_s;
+    f.v = (float)P.v;
+    f.ship_x = (float)P.ship_x;
+    f.denom_inv = (float)(1.0 / P.denom);              // 1 / (2 tanh(R/s))
+    f.inv_denom_s = (float)(P.inv_s / P.denom);        // 1 / (2 s tanh(R/s))
+    f.b0sq = (float)(P.b0 * P.b0);
+    return f;
+}
+
+GEON_D float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+GEON_D float fsqrt(float x) { return __builtin_sqrtf(x); }
+
+// tanh via one exp: tanh(u) = 1 − 2/(e^{2u} + 1), exact saturation at ±1
+GEON_D float ftanh(float u) {
+#pragma clang fp contract(fast)
+    const float e = __expf(2.0f * u);
+    return 1.0f - 2.0f * frcp(e + 1.0f);
+}
+
+// ---- Minkowski --------------------------------------------------------------
+struct MinkowskiF {
+    GEON_D static void accel(const FParams&, const float*, float* dp, float, float) {
+        dp[0] = dp[1] = dp[2] = dp[3] = 0.0f;
+    }
+    GEON_D static void reproject(const FParams&, float* y) {
+#pragma clang fp contract(fast)
+        y[4] = fsqrt(y[5] * y[5] + y[6] * y[6] + y[7] * y[7]);
+    }
+    GEON_D static int terminate(const FParams& P, const float* x) {
+#pragma clang fp contract(fast)
+        const float r2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+        return (r2 > P.esc2) ? ESCAPED : RUNNING;
+    }
+    GEON_D static bool suspicious(const FParams&, int cls, const float*) { return cls == DIVERGED; }
+    static constexpr bool uses_theta = false;
+};
+
+// ---- Schwarzschild (analytic Γ, schwarzschild.rs:61-107) --------------------
+struct SchwarzschildF {
+    static constexpr bool uses_theta = true;
+    GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
+#pragma clang fp contract(fast)
+        const float r = y[1];
+        const float pt = y[4], pr = y[5], pth = y[6], pph = y[7];
+        const float inv_r = frcp(r);
+        const float m_r = P.M * inv_r;
+        const float f = 1.0f - 2.0f * m_r;
+        const float inv_f = frcp(f);
+        const float a = m_r * inv_r;                 // M/r²
+        const float gtr = a * inv_f;                 // Γ^t_tr
+        const float grtt = a * f;                    // Γ^r_tt
+        const float rf = r * f;                      // r − 2M
+        const float cot = (fabsf(s) < 1e-8f) ? 0.0f : c * frcp(s);
+        const float pph2 = pph * pph;
+        dp[0] = -2.0f * gtr * pt * pr;
+        dp[1] = -(grtt * pt * pt - gtr * pr * pr) + rf * (pth * pth + s * s * pph2);
+        dp[2] = -2.0f * inv_r * pr * pth + s * c * pph2;
+        dp[3] = -2.0f * pph * (inv_r * pr + cot * pth);
+    }
+    GEON_D static void reproject(const FParams& P, float* y, float s) {
+#pragma clang fp contract(fast)
+        const float r = y[1];
+        const float f = 1.0f - 2.0f * P.M * frcp(r);
+        const float sp = y[5] * y[5] * frcp(f) + r * r * (y[6] * y[6] + s * s * y[7] * y[7]);
+        if (f > 0.0f && sp >= 0.0f) y[4] = fsqrt(sp * frcp(f));
+    }
+    GEON_D static int terminate(const FParams& P, const float* x) {
+        const float r = x[1];
+        if (!__builtin_isfinite(r)) return DIVERGED;
+        if (r < P.cap_r) return CAPTURED;
+        if (r > P.esc_r) return ESCAPED;
+        return RUNNING;
+    }
+    GEON_D static bool suspicious(const FParams& P, int cls, const float* y) {
+        if (cls == DIVERGED) return true;
+        if (cls == ESCAPED) return y[1] > 2.0f * P.esc_r;
+        if (cls == CAPTURED) return y[1] < 0.75f * P.cap_r;
+        return false;
+    }
+};
+
+// ---- Morris–Thorne (analytic ∂g of morris_thorne.rs:22-36) ------------------
+struct MorrisThorneF {
+    static constexpr bool uses_theta = true;
+    GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
+#pragma clang fp contract(fast)
+        const float l = y[1];
+        const float pl = y[5], pth = y[6], pph = y[7];
+        const float inv_r2 = frcp(P.b0sq + l * l);
+        const float lr = l * inv_r2;                 // Γ^θ_lθ = Γ^φ_lφ
+        const float cot = (fabsf(s) < 1e-8f) ? 0.0f : c * frcp(s);
+        const float pph2 = pph * pph;
+        dp[0] = 0.0f;
+        dp[1] = l * (pth * pth + s * s * pph2);      // −(Γ^l_θθ pθ² + Γ^l_φφ pφ²)
+        dp[2] = -2.0f * lr * pl * pth + s * c * pph2;
+        dp[3] = -2.0f * pph * (lr * pl + cot * pth);
+    }
+    GEON_D static void reproject(const FParams& P, float* y, float s) {
+#pragma clang fp contract(fast)
+        const float r2 = P.b0sq + y[1] * y[1];
+        y[4] = fsqrt(y[5] * y[5] + r2 * (y[6] * y[6] + s * s * y[7] * y[7]));
+    }
+    GEON_D static int terminate(const FParams& P, const float* x) {
+        const float l = x[1];
+        if (!__builtin_isfinite(l)) return DIVERGED;
+        return (fabsf(l) > P.esc_l) ? ESCAPED : RUNNING;
+    }
+    GEON_D static bool suspicious(const FParams& P, int cls, const float* y) {
+        if (cls == DIVERGED) return true;
+        if (cls == ESCAPED) return fabsf(y[1]) > 2.0f * P.esc_l;
+        return false;
+    }
+};
+
+// ---- Alcubierre (analytic ∂g of alcubierre.rs:33-63) ------------------------
+// With n_i = ∂_i r_s, A_i = ∂_i g_tt = 2v²ff'n_i, B_i = ∂_i g_tx = −vf'n_i and
+// g^00 = −1, g^01 = −vf, g^11 = 1 − v²f²:
+//   Q0 = 2p⁰(A·p) + 2p¹(B·p), Q1 = −A_1 (p⁰)² + 2p⁰(B·p − B_1 p¹),
+//   Qk = −A_k (p⁰)² − 2B_k p⁰p¹ (k = y, z);
+//   dp⁰ = ½(Q0 + vf Q1), dp¹ = ½(vf Q0 − (1 − v²f²) Q1), dp^k = −½ Qk.
+struct AlcubierreF {
+    static constexpr bool uses_theta = false;
+    GEON_D static void shape(const FParams& P, const float* y, float& f, float& fp, float* n) {
+#pragma clang fp contract(fast)
+        const float dx = y[1] - P.ship_x, dy = y[2], dz = y[3];
+        const float rs = fsqrt(dx * dx + dy * dy + dz * dz);
+        const float inv_rs = frcp(fmaxf(rs, 1e-30f));
+        n[0] = dx * inv_rs; n[1] = dy * inv_rs; n[2] = dz * inv_rs;
+        const float tp = ftanh((rs + P.R) * P.inv_s);
+        const float tm = ftanh((rs - P.R) * P.inv_s);
+        f = (tp - tm) * P.denom_inv;
+        fp = ((1.0f - tp * tp) - (1.0f - tm * tm)) * P.inv_denom_s;
+    }
+    GEON_D static void accel(const FParams& P, const float* y, float* dp, float, float) {
+#pragma clang fp contract(fast)
+        float f, fp, n[3];
+        shape(P, y, f, fp, n);
+        const float v = P.v;
+        const float p0 = y[4], p1 = y[5], p2 = y[6], p3 = y[7];
+        const float np = n[0] * p1 + n[1] * p2 + n[2] * p3;
+        const float Ac = 2.0f * v * v * f * fp;   // A_i = Ac n_i
+        const float Bc = -v * fp;                 // B_i = Bc n_i
+        const float Q0 = 2.0f * p0 * Ac * np + 2.0f * p1 * Bc * np;
+        const float Q1 = -Ac * n[0] * p0 * p0 + 2.0f * p0 * Bc * (np - n[0] * p1);
+        const float vf = v * f;
+        dp[0] = 0.5f * (Q0 + vf * Q1);
+        dp[1] = 0.5f * (vf * Q0 - (1.0f - vf * vf) * Q1);
+        dp[2] = 0.5f * (Ac * n[1] * p0 * p0 + 2.0f * Bc * n[1] * p0 * p1);
+        dp[3] = 0.5f * (Ac * n[2] * p0 * p0 + 2.0f * Bc * n[2] * p0 * p1);
+    }
+    GEON_D static void reproject(const FParams& P, float* y) {
+#pragma clang fp contract(fast)
+        float f, fp, n[3];
+        shape(P, y, f, fp, n);
+        const float vf = P.v * f;
+        const float a = -1.0f + vf * vf;          // g_tt
+        if (a < -1e-3f) {                         // outside any horizon of the wall
+            const float b = -vf * y[5];           // g_tx p^x
+            const float c = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
+            const float disc = b * b - a * c;
+            if (disc >= 0.0f) y[4] = (b + fsqrt(disc)) * frcp(-a);
+        }
+    }
+    GEON_D static int terminate(const FParams& P, const float* x) {
+#pragma clang fp contract(fast)
+        const float r2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+        if (!__builtin_isfinite(r2)) return DIVERGED;
+        return (r2 > P.esc2) ? ESCAPED : RUNNING;
+    }
+    GEON_D static bool suspicious(const FParams& P, int cls, const float* y) {
+#pragma clang fp contract(fast)
+        if (cls == DIVERGED) return true;
+        if (cls == ESCAPED) return y[1] * y[1] + y[2] * y[2] + y[3] * y[3] > 4.0f * P.esc2;
+        return false;
+    }
+};
+
+// FP32 RK4 step with the reference tableau; sin/cos of the stage θ computed
+// once per stage; p^t re-projected at the end of the step.
+template <class MF>
+GEON_D void rk4_step_f(const FParams& P, const float* y, float h, float* out) {
+#pragma clang fp contract(fast)
+    float acc[8], yt[8], dp[4];
+    float s = 0.0f, c = 0.0f;
+    const float hh = 0.5f * h;
+    if constexpr (MF::uses_theta) sincosf(y[2], &s, &c);
+    MF::accel(P, y, dp, s, c);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        acc[i] = y[4 + i];
+        acc[4 + i] = dp[i];
+        yt[i] = y[i] + hh * y[4 + i];
+        yt[4 + i] = y[4 + i] + hh * dp[i];
+    }
+    if constexpr (MF::uses_theta) sincosf(yt[2], &s, &c);
+    MF::accel(P, yt, dp, s, c);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float kx = yt[4 + i];
+        acc[i] += 2.0f * kx;
+        acc[4 + i] += 2.0f * dp[i];
+        yt[i] = y[i] + hh * kx;
+        yt[4 + i] = y[4 + i] + hh * dp[i];
+    }
+    if constexpr (MF::uses_theta) sincosf(yt[2], &s, &c);
+    MF::accel(P, yt, dp, s, c);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const float kx = yt[4 + i];
+        acc[i] += 2.0f * kx;
+        acc[4 + i] += 2.0f * dp[i];
+        yt[i] = y[i] + h * kx;
+        yt[4 + i] = y[4 + i] + h * dp[i];
+    }
+    if constexpr (MF::uses_theta) sincosf(yt[2], &s, &c);
+    MF::accel(P, yt, dp, s, c);
+    const float sixth = h * (1.0f / 6.0f);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        acc[i] += yt[4 + i];
+        acc[4 + i] += dp[i];
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
+    if constexpr (MF::uses_theta) {
+        const float sn = sinf(out[2]);
+        MF::reproject(P, out, sn);
+    } else {
+        MF::reproject(P, out);
+    }
+}
+
+GEON_D bool finite8f(const float* y) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) ok = ok && __builtin_isfinite(y[i]);
+    return ok;
+}
+
+// map a double metric policy to its FP32 counterpart
+template <class MP> struct FP32Of;
+template <> struct FP32Of<Minkowski> { using type = MinkowskiF; };
+template <> struct FP32Of<Schwarzschild> { using type = SchwarzschildF; };
+template <> struct FP32Of<MorrisThorne> { using type = MorrisThorneF; };
+template <> struct FP32Of<Alcubierre> { using type = AlcubierreF; };
+
+}  // namespace geon

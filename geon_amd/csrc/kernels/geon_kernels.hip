@@ -7,6 +7,7 @@
 // of an n-ray array starts at base + k·n, so consecutive lanes touch
 // consecutive addresses.
 #include "geon_device.hpp"
+#include "geon_sched.hpp"
 
 using namespace geon;
 
@@ -95,11 +96,11 @@ GEON_KERNELS(mt, MorrisThorne)
 // render.rs:96-163: colour per sub-ray, f32 average, majority class with
 // ties captured > diverged > escaped, integer per-pixel step average, max
 // finite residual; statistics summed over pixels.
-extern "C" __global__ __launch_bounds__(256) void geon_render_finalize(
-    uint32_t slot, uint64_t n_pix, const int32_t* __restrict__ cls, const uint32_t* __restrict__ steps,
-    const double* __restrict__ resid, const double* __restrict__ dir, float* __restrict__ rgb,
-    int32_t* __restrict__ cls_pix, double* __restrict__ dir_pix, geon_dstats* __restrict__ stats) {
-    const geon_dframe& F = c_frames[slot];
+__device__ void finalize_pixels(const geon_dframe& F, uint64_t n_pix, const int32_t* __restrict__ cls,
+                                const uint32_t* __restrict__ steps, const double* __restrict__ resid,
+                                const double* __restrict__ dir, float* __restrict__ rgb,
+                                int32_t* __restrict__ cls_pix, double* __restrict__ dir_pix,
+                                geon_dstats* __restrict__ stats) {
     const uint64_t pix = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t spa2 = F.spa * F.spa;
     const uint64_t n_rays = n_pix * spa2;
@@ -181,6 +182,27 @@ extern "C" __global__ __launch_bounds__(256) void geon_render_finalize(
         atomicAdd(&stats->steps_pix_sum, red[5][0]);
         atomicMax(&stats->max_resid_bits, red[6][0]);
     }
+}
+
+extern "C" __global__ __launch_bounds__(256) void geon_render_finalize(
+    uint32_t slot, uint64_t n_pix, const int32_t* __restrict__ cls, const uint32_t* __restrict__ steps,
+    const double* __restrict__ resid, const double* __restrict__ dir, float* __restrict__ rgb,
+    int32_t* __restrict__ cls_pix, double* __restrict__ dir_pix, geon_dstats* __restrict__ stats) {
+    finalize_pixels(c_frames[slot], n_pix, cls, steps, resid, dir, rgb, cls_pix, dir_pix, stats);
+}
+
+// the same reduction for a scheduled camera job (launched with 256 threads)
+extern "C" __global__ __launch_bounds__(256) void geon_job_finalize(const geon_djob* __restrict__ J) {
+    const uint64_t n_pix = (uint64_t)J->F.W * J->F.H;
+    finalize_pixels(J->F, n_pix, J->cls, J->steps, J->resid, J->dir, J->rgb, J->cls_pix, J->dir_pix,
+                    J->stats);
+}
+
+// Zero n 64-bit words (counters, cursors, statistics) without a memset node.
+extern "C" __global__ void geon_zero_u64(unsigned long long* p, uint64_t n) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
 }
 
 // Zero a statistics block (used instead of a memset node so the sequence
@@ -378,3 +400,20 @@ extern "C" __global__ __launch_bounds__(64) void geon_bisect_serial(
     alpha_c[k] = a;
     b_c[k] = r * sin(a) / sqrt(f);
 }
+
+// ---- persistent scheduler instantiations -------------------------------------
+#define GEON_SCHED(NAME, MP, T, TN, RPL)                                                         \
+    extern "C" __global__ __launch_bounds__(256) void geon_sched_##NAME##_##TN##_r##RPL(        \
+        geon_sched_args A) {                                                                    \
+        sched_run<MP, T, RPL>(A);                                                               \
+    }
+#define GEON_SCHED_ALL(NAME, MP)                                                                \
+    GEON_SCHED(NAME, MP, double, f64, 1)                                                        \
+    GEON_SCHED(NAME, MP, double, f64, 2)                                                        \
+    GEON_SCHED(NAME, MP, float, f32, 1)                                                         \
+    GEON_SCHED(NAME, MP, float, f32, 2)
+
+GEON_SCHED_ALL(mink, Minkowski)
+GEON_SCHED_ALL(schw, Schwarzschild)
+GEON_SCHED_ALL(alc, Alcubierre)
+GEON_SCHED_ALL(mt, MorrisThorne)

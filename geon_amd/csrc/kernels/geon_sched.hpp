@@ -1,0 +1,333 @@
+// geon_sched.hpp — persistent ray scheduler for gfx950.
+//
+// A fixed grid of lanes drains a queue of rays. Each lane holds RPL rays in
+// registers, integrates them step by step, retires a ray the moment it
+// terminates (writing its outputs) and refills the freed slot from the
+// queue. Refills happen at wave-uniform points every `refill` steps: the
+// wave counts its empty slots with a ballot and one lane claims that many
+// queue positions with a compare-and-swap that never moves the cursor past
+// the end, so positions are neither lost nor handed out twice. Ragged ray
+// lengths across many frames therefore share one launch without idle lanes
+// until the queue runs dry.
+//
+// Quantum mode (serving): a step budget bounds each launch; rays still in
+// flight are written to a slot pool and resumed by the next launch, so
+// requests admitted between launches join the rays already running.
+#pragma once
+#include "geon_fp32.hpp"
+
+namespace geon {
+
+GEON_D uint32_t find_job(const unsigned long long* base, uint32_t n, unsigned long long g) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= g) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// initial state of a job's ray (camera.rs:113-160 or caller-supplied)
+GEON_D void job_ray(const geon_djob& J, unsigned long long local, double* y) {
+    if (J.gen == 1) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) y[c] = J.y0[c * J.n_rays + local];
+        return;
+    }
+    const geon_dframe& F = J.F;
+    const uint32_t spa = F.spa, spa2 = spa * spa;
+    const unsigned long long pix = local / spa2;
+    const uint32_t sub = (uint32_t)(local - pix * spa2);
+    const double du = ((double)(sub % spa) + 0.5) * F.inv_spa;
+    const double dv = ((double)(sub / spa) + 0.5) * F.inv_spa;
+    camera_ray(F, (uint32_t)(pix % F.W), (uint32_t)(pix / F.W), du, dv, y);
+}
+
+// Impact parameter b = L/E of a ray in a static spherically symmetric chart
+// (E = −g_tt p^t, L² = (g_θθ p^θ)² + (g_φφ p^φ)²/sin²θ); rays with
+// |b − crit_b| < fp64_band orbit close to the photon sphere (throat) and
+// are integrated in FP64.
+template <class MP>
+GEON_D bool near_photon_orbit(const geon_dframe& F, const double* y) {
+    if (!(F.fp64_band > 0.0)) return false;
+    if constexpr (std::is_same<MP, Schwarzschild>::value || std::is_same<MP, MorrisThorne>::value) {
+        double g[16];
+        MP::g(F.m, y, g);
+        const double s = sin(y[2]);
+        const double E = -g[0] * y[4];
+        const double Lt = g[10] * y[6];
+        const double Lp = (s != 0.0) ? g[15] * y[7] / s : 0.0;
+        const double b = sqrt(Lt * Lt + Lp * Lp) / E;
+        return fabs(b - F.crit_b) < F.fp64_band;
+    } else {
+        return false;
+    }
+}
+
+// Exact early decision for capture predicates (bisection): in Schwarzschild
+// a null ray moving outward beyond r = 3M never turns back, one moving
+// inward inside r = 3M never turns around.
+GEON_D int photon_sphere_decision(const geon_dmetric& P, double r, double pr) {
+    const double r3 = 3.0 * P.M;
+    if (r > r3 && pr > 0.0) return ESCAPED;
+    if (r < r3 && pr < 0.0) return CAPTURED;
+    return RUNNING;
+}
+
+template <class MP>
+GEON_D void write_result(const geon_djob& J, unsigned long long local, int cls, uint32_t n,
+                         const double* y) {
+    if (J.cls) J.cls[local] = cls;
+    if (J.steps) J.steps[local] = n;
+    if (J.resid) J.resid[local] = null_residual<MP>(J.F.m, y);
+    if (J.dir) {
+        double d[3] = {0.0, 0.0, 0.0};
+        if (cls == ESCAPED) direction<MP>(y, d);
+        J.dir[local] = d[0];
+        J.dir[J.n_rays + local] = d[1];
+        J.dir[2 * J.n_rays + local] = d[2];
+    }
+    if (J.yfin) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) J.yfin[c * J.n_rays + local] = y[c];
+    }
+    if (J.done) atomicAdd(J.done, 1ull);
+}
+
+// initial state of ray gid in precision T; returns false when the ray is
+// routed to the FP64 list instead
+template <class MP, class T>
+GEON_D bool init_ray(const geon_djob& J, unsigned long long gid,
+                                                   const geon_sched_args& A, T* y) {
+    double yd[8];
+    job_ray(J, gid - J.ray_base, yd);
+    if constexpr (std::is_same<T, float>::value) {
+        if (J.F.precision == 1 && near_photon_orbit<MP>(J.F, yd)) {
+            const unsigned long long q = atomicAdd(A.fb_count, 1ull);
+            if (q < A.fb_cap) A.fb_list[q] = gid;
+            if (J.fp64) atomicAdd(J.fp64, 1ull);
+            return false;
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) y[c] = (T)yd[c];
+    return true;
+}
+
+template <class T>
+GEON_D void push_fallback(const geon_sched_args& A, const geon_djob& J, unsigned long long gid) {
+    const unsigned long long q = atomicAdd(A.fb_count, 1ull);
+    if (q < A.fb_cap) A.fb_list[q] = gid;
+    if (J.fp64) atomicAdd(J.fp64, 1ull);
+}
+
+// per-precision operations
+template <class MP, class T>
+struct Prec;
+
+template <class MP>
+struct Prec<MP, double> {
+    using Params = geon_dmetric;
+    GEON_D static Params params(const geon_dmetric& m) { return m; }
+    GEON_D static int terminate(const Params& P, const double* y) { return MP::terminate(P, y); }
+    GEON_D static void step(const Params& P, const double* y, double h, double* out) {
+        rk4_step<MP>(P, y, h, out);
+    }
+    GEON_D static bool finite(const double* y) { return finite8(y); }
+    GEON_D static bool suspicious(const Params&, int, const double*) { return false; }
+};
+
+template <class MP>
+struct Prec<MP, float> {
+    using MF = typename FP32Of<MP>::type;
+    using Params = FParams;
+    GEON_D static Params params(const geon_dmetric& m) { return fparams(m); }
+    GEON_D static int terminate(const Params& P, const float* y) { return MF::terminate(P, y); }
+    GEON_D static void step(const Params& P, const float* y, float h, float* out) {
+        rk4_step_f<MF>(P, y, h, out);
+    }
+    GEON_D static bool finite(const float* y) { return finite8f(y); }
+    GEON_D static bool suspicious(const Params& P, int cls, const float* y) {
+        return MF::suspicious(P, cls, y);
+    }
+};
+
+template <class MP, class T, int RPL>
+__device__ void sched_run(const geon_sched_args& A) {
+    using PR = Prec<MP, T>;
+    using Params = typename PR::Params;
+    const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wl = __lane_id();
+    const unsigned long long end = A.end_ptr ? *A.end_ptr : A.end;
+
+    T y[RPL][8];
+    bool act[RPL];
+    unsigned long long g[RPL];
+    uint32_t n[RPL], jb[RPL], maxs[RPL];
+    T h[RPL];
+    int early[RPL];
+    Params P[RPL];
+    unsigned long long executed = 0;
+
+    auto activate = [&](int r, unsigned long long gid, uint32_t steps_done, bool resume) {
+        const uint32_t k = find_job(A.job_base, A.n_jobs, gid);
+        const geon_djob& J = A.jobs[k];
+        g[r] = gid;
+        jb[r] = k;
+        n[r] = steps_done;
+        maxs[r] = J.F.max_steps;
+        h[r] = (T)J.F.step;
+        early[r] = J.early_exit;
+        P[r] = PR::params(J.F.m);
+        act[r] = true;
+        if (!resume) act[r] = init_ray<MP, T>(J, gid, A, y[r]);
+    };
+
+    auto finish = [&](int r, int cls) {
+        act[r] = false;
+        const geon_djob& J = A.jobs[jb[r]];
+        if (PR::suspicious(P[r], cls, y[r])) {
+            push_fallback<T>(A, J, g[r]);
+            return;
+        }
+        double yd[8];
+#pragma unroll
+        for (int c = 0; c < 8; ++c) yd[c] = (double)y[r][c];
+        write_result<MP>(J, g[r] - J.ray_base, cls, n[r], yd);
+    };
+
+    // resume rays left in flight by the previous quantum
+#pragma unroll
+    for (int r = 0; r < RPL; ++r) {
+        act[r] = false;
+        if (A.slot_g) {
+            const uint32_t s = lane * RPL + r;
+            const unsigned long long gid = A.slot_g[s];
+            if (gid != ~0ull) {
+                activate(r, gid, A.slot_n[s], true);
+                const T* sy = (const T*)A.slot_y;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) y[r][c] = sy[(size_t)c * A.n_slots + s];
+            }
+        }
+    }
+
+    bool exhausted = false;
+    uint32_t budget = A.quantum;
+    for (;;) {
+        // ---- refill point (all lanes of the wave converged here) ----
+        if (!exhausted) {
+            unsigned long long masks[RPL], orm = 0;
+            uint32_t total = 0;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                masks[r] = __ballot(!act[r]);
+                orm |= masks[r];
+                total += __popcll(masks[r]);
+            }
+            if (total > 0) {
+                const int leader = __ffsll((long long)orm) - 1;
+                unsigned long long base = 0, take = 0;
+                if (wl == (uint32_t)(leader < 0 ? 0 : leader)) {
+                    unsigned long long cur =
+                        __hip_atomic_load(A.cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    for (;;) {
+                        if (cur >= end) {
+                            take = 0;
+                            break;
+                        }
+                        take = end - cur < total ? end - cur : total;
+                        const unsigned long long prev = atomicCAS(A.cursor, cur, cur + take);
+                        if (prev == cur) break;
+                        cur = prev;
+                    }
+                    base = cur;
+                }
+                base = __shfl(base, leader < 0 ? 0 : leader);
+                take = __shfl(take, leader < 0 ? 0 : leader);
+                if (take < total) exhausted = true;
+                uint32_t before = 0;
+#pragma unroll
+                for (int r = 0; r < RPL; ++r) {
+                    const unsigned long long below = masks[r] & ((1ull << wl) - 1ull);
+                    const uint32_t off = before + (uint32_t)__popcll(below);
+                    if (!act[r] && off < take) {
+                        const unsigned long long pos = base + off;
+                        activate(r, A.list ? A.list[pos] : pos, 0, false);
+                    }
+                    before += __popcll(masks[r]);
+                }
+            }
+        }
+        bool any = false;
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) any = any || act[r];
+        if (__ballot(any) == 0ull) {
+            if (exhausted) break;
+            continue;  // every claimed ray went to the FP64 list; claim again
+        }
+        if (A.quantum && budget == 0) break;
+        uint32_t chunk = A.refill;
+        if (A.quantum && chunk > budget) chunk = budget;
+
+        // ---- integrate ----
+        for (uint32_t it = 0; it < chunk; ++it) {
+            bool live = false;
+#pragma unroll
+            for (int r = 0; r < RPL; ++r) {
+                if (!act[r]) continue;
+                live = true;
+                if (n[r] >= maxs[r]) {
+                    finish(r, MAXSTEPS);
+                    continue;
+                }
+                int t = PR::terminate(P[r], y[r]);
+                if (t == RUNNING && early[r]) {
+                    if constexpr (std::is_same<MP, Schwarzschild>::value) {
+                        const geon_dmetric& Pm = A.jobs[jb[r]].F.m;
+                        t = photon_sphere_decision(Pm, (double)y[r][1], (double)y[r][5]);
+                    }
+                }
+                if (t != RUNNING) {
+                    finish(r, t);
+                    continue;
+                }
+                T yn[8];
+                PR::step(P[r], y[r], h[r], yn);
+                if (!PR::finite(yn)) {
+                    finish(r, DIVERGED);
+                    continue;
+                }
+#pragma unroll
+                for (int c = 0; c < 8; ++c) y[r][c] = yn[c];
+                ++n[r];
+                ++executed;
+            }
+            if (!live) break;
+        }
+        if (A.quantum) budget -= chunk;
+    }
+
+    // park rays still in flight
+    if (A.slot_g) {
+#pragma unroll
+        for (int r = 0; r < RPL; ++r) {
+            const uint32_t s = lane * RPL + r;
+            if (act[r]) {
+                A.slot_g[s] = g[r];
+                A.slot_n[s] = n[r];
+                T* sy = (T*)A.slot_y;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) sy[(size_t)c * A.n_slots + s] = y[r][c];
+            } else {
+                A.slot_g[s] = ~0ull;
+            }
+        }
+    }
+    // one atomic per wave for the executed-step counter
+    for (int off = 32; off > 0; off >>= 1) executed += __shfl_xor(executed, off);
+    if (wl == 0 && A.ray_steps) atomicAdd(A.ray_steps, executed);
+}
+
+}  // namespace geon

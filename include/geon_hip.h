@@ -101,6 +101,10 @@ typedef struct geon_render_cfg {
     float    diverged_rgb[3];
     uint32_t show_stars;
     int32_t  precision;
+    /* FP32-mixed: rays whose impact parameter is within this distance of the
+     * photon orbit (3√3 M for Schwarzschild, b0 for Morris–Thorne) are
+     * integrated in FP64; 0 selects 1e-3 of the mass (throat radius). */
+    double   fp64_band;
 } geon_render_cfg_t;
 
 /* RenderStats (render.rs:45-51) plus counters the reference does not keep */
@@ -200,6 +204,51 @@ int  geon_curvature_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const doub
 int  geon_shadow_bisect(geon_ctx_t* ctx, const double* mass, const double* r_obs,
                         size_t n, double step, uint32_t max_steps, uint32_t iters,
                         double* alpha_c, double* b_c);
+
+/* ---- persistent scheduler (no reference counterpart: the reference runs one
+ * rayon task per pixel, render.rs:73-75; these batch many frames and ray sets
+ * of ragged length into persistent gfx950 launches) ------------------------ */
+
+/* Launch shape of a scheduler kernel; zero fields take the tuned defaults. */
+typedef struct geon_sched_cfg {
+    uint32_t block;           /* threads per workgroup (64, 128 or 256)        */
+    uint32_t waves_per_simd;  /* resident waves per SIMD the grid is sized for */
+    uint32_t rays_per_lane;   /* 1 or 2 rays integrated per lane               */
+    uint32_t refill;          /* RK4 steps between refill points               */
+} geon_sched_cfg_t;
+
+typedef struct geon_batch geon_batch_t;
+
+int  geon_batch_create(geon_ctx_t* ctx, geon_batch_t** out);
+void geon_batch_destroy(geon_batch_t* batch);
+/* Add a camera frame (render semantics). Outputs are device pointers, any may
+ * be NULL; layout as geon_render. cfg->precision selects FP64 or FP32-mixed. */
+int  geon_batch_add_frame(geon_batch_t* batch, const geon_metric_t* metric,
+                          const geon_camera_t* cam, const geon_render_cfg_t* cfg,
+                          float* rgb, int32_t* cls_per_pixel, double* dir_per_pixel,
+                          uint32_t* frame_id);
+/* Add n explicit rays (device SoA 8×n initial states, trace semantics).
+ * Outputs are device pointers, any may be NULL. early_exit (Schwarzschild
+ * only) ends a ray as soon as it is outgoing beyond r = 3M (escaped) or
+ * ingoing inside it (captured), which decides capture without the tail. */
+int  geon_batch_add_rays(geon_batch_t* batch, const geon_metric_t* metric,
+                         const double* y0_soa, size_t n, double step, uint32_t max_steps,
+                         int32_t precision, int32_t early_exit, int32_t* cls, uint32_t* steps,
+                         double* y_final_soa, double* null_resid, double* dir_soa,
+                         uint32_t* job_id);
+/* Enqueue every added job on `stream` (NULL: the context stream); metric
+ * kinds run concurrently on forked streams joined back into `stream`. */
+int  geon_batch_run(geon_batch_t* batch, const geon_sched_cfg_t* tune, void* stream);
+/* Statistics of a frame after the run (synchronises). */
+int  geon_batch_frame_stats(geon_batch_t* batch, uint32_t frame_id, geon_render_stats_t* out);
+/* Totals after the run (synchronises): executed RK4 ray-steps, rays re-traced
+ * in FP64, device time between the run's first and last kernel. */
+int  geon_batch_totals(geon_batch_t* batch, uint64_t* ray_steps, uint64_t* fp64_rays,
+                       double* device_ms);
+/* Time candidate launch shapes on a calibration frame of this metric kind
+ * and precision; the winner becomes the context default and is returned. */
+int  geon_autotune(geon_ctx_t* ctx, int32_t metric_kind, int32_t precision,
+                   geon_sched_cfg_t* best);
 
 /* write_png's per-channel map: clamp(0,1)^(1/2.2)·255, rounded (render.rs:175-179). */
 int  geon_tonemap_u8(const float* rgb, size_t n, uint8_t* out);

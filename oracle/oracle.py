@@ -38,7 +38,7 @@ class RenderCfg(C.Structure):
     _fields_ = [("step", C.c_double), ("max_steps", C.c_uint32),
                 ("samples_per_axis", C.c_uint32), ("captured_rgb", C.c_float * 3),
                 ("diverged_rgb", C.c_float * 3), ("show_stars", C.c_uint32),
-                ("precision", C.c_int32)]
+                ("precision", C.c_int32), ("fp64_band", C.c_double)]
 
 
 class RenderStats(C.Structure):
