@@ -102,12 +102,33 @@ SIGNATURES = [
 _lib = None
 
 
+def _one_hip_runtime():
+    """A process must use a single HIP runtime. PyTorch wheels bundle their own
+    libamdhip64.so.7; the dynamic linker binds every later request for that
+    soname to whichever copy was loaded first. When torch is installed, load
+    its copy (without importing torch) so that libgeon_hip.so and torch share
+    it whatever the import order."""
+    import importlib.util
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        spec = None
+    if spec is None or not spec.submodule_search_locations:
+        return
+    for d in spec.submodule_search_locations:
+        cand = os.path.join(d, "lib", "libamdhip64.so")
+        if os.path.exists(cand):
+            C.CDLL(cand, mode=C.RTLD_GLOBAL)
+            return
+
+
 def lib():
     """Load libgeon_hip.so; raise if it has not been built."""
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m geon_amd.build`")
+        _one_hip_runtime()
         L = C.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             fn = getattr(L, name)
