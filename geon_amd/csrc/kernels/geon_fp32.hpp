@@ -9,9 +9,11 @@
 // (future-directed root), which keeps FP32 drift out of the
 // energy-carrying component. Rays whose outcome is sensitive to rounding
 // are handed to the FP64 path: a priori those whose impact parameter lies
-// within a band of a photon orbit, a posteriori those that end in a state
-// a smooth fixed-step trajectory cannot reach (NaN, or a final step that
-// jumped far past the escape sphere or deep into the capture region).
+// within a band of a photon orbit; a posteriori those that end diverged or
+// flung far past the escape sphere, Schwarzschild captures (the reference
+// decides them at a near-singular stage), and those whose trajectory passed
+// within |sin θ| < 0.02 of the coordinate pole or dipped inside 3.1M (0.1 b0
+// at a wormhole throat), where FP32 errors are amplified.
 #pragma once
 #include "geon_device.hpp"
 
@@ -22,7 +24,12 @@ struct FParams {
     float M, cap_r, esc_r, esc2, esc_l;
     float R, inv_s, v, ship_x, inv_denom_s, denom_inv;
     float b0sq;
+    float near_orbit;   // a ray dipping below this radius (|l|) is re-traced in FP64
 };
+
+// a ray whose |sin θ| drops below this passes the coordinate pole, where the
+// spherical chart is ill-conditioned in FP32
+#define GEON_POLE_SIN 0.02f
 
 GEON_D FParams fparams(const geon_dmetric& P) {
     FParams f;
@@ -38,6 +45,7 @@ GEON_D FParams fparams(const geon_dmetric& P) {
     f.denom_inv = (float)(1.0 / P.denom);              // 1 / (2 tanh(R/s))
     f.inv_denom_s = (float)(P.inv_s / P.denom);        // 1 / (2 s tanh(R/s))
     f.b0sq = (float)(P.b0 * P.b0);
+    f.near_orbit = P.kind == 1 ? (float)(3.1 * P.M) : (P.kind == 3 ? (float)(0.1 * P.b0) : 0.0f);
     return f;
 }
 
@@ -65,7 +73,10 @@ struct MinkowskiF {
         const float r2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
         return (r2 > P.esc2) ? ESCAPED : RUNNING;
     }
-    GEON_D static bool suspicious(const FParams&, int cls, const float*) { return cls == DIVERGED; }
+    GEON_D static bool suspicious(const FParams&, int cls, const float*, float, float) {
+        return cls == DIVERGED;
+    }
+    GEON_D static float radius(const float*) { return 1e30f; }
     static constexpr bool uses_theta = false;
 };
 
@@ -105,12 +116,15 @@ struct SchwarzschildF {
         if (r > P.esc_r) return ESCAPED;
         return RUNNING;
     }
-    GEON_D static bool suspicious(const FParams& P, int cls, const float* y) {
-        if (cls == DIVERGED) return true;
-        if (cls == ESCAPED) return y[1] > 2.0f * P.esc_r;
-        if (cls == CAPTURED) return y[1] < 0.75f * P.cap_r;
-        return false;
+    // Capture is decided by FP64 too: whether the reference's fixed step
+    // crosses r = 2.01M smoothly or has a stage land on r ≈ 2M (and fling the
+    // ray out) depends on FP64 rounding at that near-singular stage.
+    GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float min_s, float min_r) {
+        if (cls == DIVERGED || cls == CAPTURED) return true;
+        if (cls == ESCAPED && y[1] > 2.0f * P.esc_r) return true;
+        return min_s < GEON_POLE_SIN || min_r < P.near_orbit;
     }
+    GEON_D static float radius(const float* y) { return y[1]; }
 };
 
 // ---- Morris–Thorne (analytic ∂g of morris_thorne.rs:22-36) ------------------
@@ -139,11 +153,12 @@ struct MorrisThorneF {
         if (!__builtin_isfinite(l)) return DIVERGED;
         return (fabsf(l) > P.esc_l) ? ESCAPED : RUNNING;
     }
-    GEON_D static bool suspicious(const FParams& P, int cls, const float* y) {
+    GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float min_s, float min_r) {
         if (cls == DIVERGED) return true;
-        if (cls == ESCAPED) return fabsf(y[1]) > 2.0f * P.esc_l;
-        return false;
+        if (cls == ESCAPED && fabsf(y[1]) > 2.0f * P.esc_l) return true;
+        return min_s < GEON_POLE_SIN || min_r < P.near_orbit;
     }
+    GEON_D static float radius(const float* y) { return fabsf(y[1]); }
 };
 
 // ---- Alcubierre (analytic ∂g of alcubierre.rs:33-63) ------------------------
@@ -201,18 +216,20 @@ struct AlcubierreF {
         if (!__builtin_isfinite(r2)) return DIVERGED;
         return (r2 > P.esc2) ? ESCAPED : RUNNING;
     }
-    GEON_D static bool suspicious(const FParams& P, int cls, const float* y) {
+    GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float, float) {
 #pragma clang fp contract(fast)
         if (cls == DIVERGED) return true;
         if (cls == ESCAPED) return y[1] * y[1] + y[2] * y[2] + y[3] * y[3] > 4.0f * P.esc2;
         return false;
     }
+    GEON_D static float radius(const float*) { return 1e30f; }
 };
 
 // FP32 RK4 step with the reference tableau; sin/cos of the stage θ computed
 // once per stage; p^t re-projected at the end of the step.
+// Returns |sin θ| of the new state (1 for Cartesian charts).
 template <class MF>
-GEON_D void rk4_step_f(const FParams& P, const float* y, float h, float* out) {
+GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out) {
 #pragma clang fp contract(fast)
     float acc[8], yt[8], dp[4];
     float s = 0.0f, c = 0.0f;
@@ -259,8 +276,10 @@ GEON_D void rk4_step_f(const FParams& P, const float* y, float h, float* out) {
     if constexpr (MF::uses_theta) {
         const float sn = sinf(out[2]);
         MF::reproject(P, out, sn);
+        return fabsf(sn);
     } else {
         MF::reproject(P, out);
+        return 1.0f;
     }
 }
 

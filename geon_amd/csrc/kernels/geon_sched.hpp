@@ -131,11 +131,13 @@ struct Prec<MP, double> {
     using Params = geon_dmetric;
     GEON_D static Params params(const geon_dmetric& m) { return m; }
     GEON_D static int terminate(const Params& P, const double* y) { return MP::terminate(P, y); }
-    GEON_D static void step(const Params& P, const double* y, double h, double* out) {
+    GEON_D static float step(const Params& P, const double* y, double h, double* out) {
         rk4_step<MP>(P, y, h, out);
+        return 1.0f;
     }
+    GEON_D static float radius(const double*) { return 1e30f; }
     GEON_D static bool finite(const double* y) { return finite8(y); }
-    GEON_D static bool suspicious(const Params&, int, const double*) { return false; }
+    GEON_D static bool suspicious(const Params&, int, const double*, float, float) { return false; }
 };
 
 template <class MP>
@@ -144,12 +146,13 @@ struct Prec<MP, float> {
     using Params = FParams;
     GEON_D static Params params(const geon_dmetric& m) { return fparams(m); }
     GEON_D static int terminate(const Params& P, const float* y) { return MF::terminate(P, y); }
-    GEON_D static void step(const Params& P, const float* y, float h, float* out) {
-        rk4_step_f<MF>(P, y, h, out);
+    GEON_D static float step(const Params& P, const float* y, float h, float* out) {
+        return rk4_step_f<MF>(P, y, h, out);
     }
+    GEON_D static float radius(const float* y) { return MF::radius(y); }
     GEON_D static bool finite(const float* y) { return finite8f(y); }
-    GEON_D static bool suspicious(const Params& P, int cls, const float* y) {
-        return MF::suspicious(P, cls, y);
+    GEON_D static bool suspicious(const Params& P, int cls, const float* y, float min_s, float min_r) {
+        return MF::suspicious(P, cls, y, min_s, min_r);
     }
 };
 
@@ -167,6 +170,7 @@ __device__ void sched_run(const geon_sched_args& A) {
     uint32_t n[RPL], jb[RPL], maxs[RPL];
     T h[RPL];
     int early[RPL];
+    float min_s[RPL], min_r[RPL];   // FP32 conditioning trackers
     Params P[RPL];
     unsigned long long executed = 0;
 
@@ -180,6 +184,8 @@ __device__ void sched_run(const geon_sched_args& A) {
         h[r] = (T)J.F.step;
         early[r] = J.early_exit;
         P[r] = PR::params(J.F.m);
+        min_s[r] = 1.0f;
+        min_r[r] = 1e30f;
         act[r] = true;
         if (!resume) act[r] = init_ray<MP, T>(J, gid, A, y[r]);
     };
@@ -187,7 +193,7 @@ __device__ void sched_run(const geon_sched_args& A) {
     auto finish = [&](int r, int cls) {
         act[r] = false;
         const geon_djob& J = A.jobs[jb[r]];
-        if (PR::suspicious(P[r], cls, y[r])) {
+        if (PR::suspicious(P[r], cls, y[r], min_s[r], min_r[r])) {
             push_fallback<T>(A, J, g[r]);
             return;
         }
@@ -294,13 +300,17 @@ __device__ void sched_run(const geon_sched_args& A) {
                     continue;
                 }
                 T yn[8];
-                PR::step(P[r], y[r], h[r], yn);
+                const float sn = PR::step(P[r], y[r], h[r], yn);
                 if (!PR::finite(yn)) {
                     finish(r, DIVERGED);
                     continue;
                 }
 #pragma unroll
                 for (int c = 0; c < 8; ++c) y[r][c] = yn[c];
+                if constexpr (std::is_same<T, float>::value) {
+                    min_s[r] = fminf(min_s[r], sn);
+                    min_r[r] = fminf(min_r[r], PR::radius(yn));
+                }
                 ++n[r];
                 ++executed;
             }
