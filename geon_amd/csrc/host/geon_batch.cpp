@@ -125,7 +125,7 @@ static int sched_launch(geon_ctx* c, int kind, int f32, const geon_sched_cfg_t& 
     const uint32_t rpl = cfg.rays_per_lane;
     uint64_t lanes = (uint64_t)c->n_cu * 4ull * cfg.waves_per_simd * 64ull;
     const uint64_t need = ((rays + rpl - 1) / rpl + 63) / 64 * 64;
-    if (rays > 0 && need < lanes) lanes = need;
+    if (rays > 0 && need < lanes) lanes = need;   // rays == 0: unknown, full GPU
     if (lanes < 64) lanes = 64;
     lanes = (lanes + cfg.block - 1) / cfg.block * cfg.block;
     args.refill = cfg.refill;
@@ -196,7 +196,6 @@ int batch_run_on(geon_batch* b, const geon_sched_cfg_t* tune, hipStream_t user) 
             GEON_TRY(sched_launch(c, k, 1, cfg, a, b->q_rays[k][1], s));
             // FP64 pass over the rays the FP32 pass handed back
             geon_sched_cfg_t cf = resolve_cfg(c, k, 0, nullptr);
-            cf.rays_per_lane = 1;
             geon_sched_args f;
             memset(&f, 0, sizeof f);
             f.jobs = b->d_jobs[k][1];
@@ -206,8 +205,8 @@ int batch_run_on(geon_batch* b, const geon_sched_cfg_t* tune, hipStream_t user) 
             f.end_ptr = ck + C_FBCOUNT;
             f.list = b->d_fb_list[k];
             f.ray_steps = ck + C_STEPSFB;
-            const uint64_t guess = b->q_rays[k][1] / 64 + 4096;
-            GEON_TRY(sched_launch(c, k, 0, cf, f, guess, s));
+            // the list length is only known on the device: size for the full GPU
+            GEON_TRY(sched_launch(c, k, 0, cf, f, 0, s));
         }
         if (b->q_rays[k][0]) {
             geon_sched_cfg_t cfg = resolve_cfg(c, k, 0, tune);

@@ -235,8 +235,8 @@ int ctx_create(int device, const char* code_object, geon_ctx** out) {
         c->n_cu = ncu;
     // launch-shape defaults until geon_autotune replaces them
     for (int k = 0; k < 4; ++k) {
-        c->tuned[k][0] = {256, 2, 1, 32};
-        c->tuned[k][1] = {256, 4, 1, 32};
+        c->tuned[k][0] = {256, 4, 2, 64};
+        c->tuned[k][1] = {256, 4, 2, 64};
     }
     hipEventCreate(&c->t0);
     hipEventCreate(&c->t1);

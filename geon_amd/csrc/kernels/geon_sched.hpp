@@ -5,8 +5,8 @@
 // terminates (writing its outputs) and refills the freed slot from the
 // queue. Refills happen at wave-uniform points every `refill` steps: the
 // wave counts its empty slots with a ballot and one lane claims that many
-// queue positions with a compare-and-swap that never moves the cursor past
-// the end, so positions are neither lost nor handed out twice. Ragged ray
+// queue positions with a single atomicAdd, so every position is handed out
+// exactly once. Ragged ray
 // lengths across many frames therefore share one launch without idle lanes
 // until the queue runs dry.
 //
@@ -223,6 +223,10 @@ __device__ void sched_run(const geon_sched_args& A) {
     uint32_t budget = A.quantum;
     for (;;) {
         // ---- refill point (all lanes of the wave converged here) ----
+        // A wave claims when a quarter of its slots are empty (or all are):
+        // one atomicAdd per claim. A claim may run past `end`; positions at or
+        // beyond it are simply not used, and between serving quanta the host
+        // clamps the cursor back to `end` before admitting more work.
         if (!exhausted) {
             unsigned long long masks[RPL], orm = 0;
             uint32_t total = 0;
@@ -232,26 +236,14 @@ __device__ void sched_run(const geon_sched_args& A) {
                 orm |= masks[r];
                 total += __popcll(masks[r]);
             }
-            if (total > 0) {
+            const uint32_t slots = 64u * RPL;
+            if (total > 0 && (4u * total >= slots || total == slots)) {
                 const int leader = __ffsll((long long)orm) - 1;
-                unsigned long long base = 0, take = 0;
-                if (wl == (uint32_t)(leader < 0 ? 0 : leader)) {
-                    unsigned long long cur =
-                        __hip_atomic_load(A.cursor, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    for (;;) {
-                        if (cur >= end) {
-                            take = 0;
-                            break;
-                        }
-                        take = end - cur < total ? end - cur : total;
-                        const unsigned long long prev = atomicCAS(A.cursor, cur, cur + take);
-                        if (prev == cur) break;
-                        cur = prev;
-                    }
-                    base = cur;
-                }
-                base = __shfl(base, leader < 0 ? 0 : leader);
-                take = __shfl(take, leader < 0 ? 0 : leader);
+                unsigned long long base = 0;
+                if (wl == (uint32_t)leader) base = atomicAdd(A.cursor, (unsigned long long)total);
+                base = __shfl(base, leader);
+                const unsigned long long take =
+                    base >= end ? 0ull : (end - base < total ? end - base : (unsigned long long)total);
                 if (take < total) exhausted = true;
                 uint32_t before = 0;
 #pragma unroll
