@@ -225,16 +225,33 @@ struct AlcubierreF {
     GEON_D static float radius(const float*) { return 1e30f; }
 };
 
-// FP32 RK4 step with the reference tableau; sin/cos of the stage θ computed
-// once per stage; p^t re-projected at the end of the step.
-// Returns |sin θ| of the new state (1 for Cartesian charts).
+// (sin, cos) of θ + δ from (sin θ, cos θ): rotation by δ with sin δ, cos δ
+// from their Taylor series (|δ| ≤ 0.25: truncation below 2e-8), falling back
+// to sincosf for a large stage increment.
+GEON_D void rotate_sc(float s, float c, float theta_new, float d, float& so, float& co) {
+#pragma clang fp contract(fast)
+    if (fabsf(d) > 0.25f) {
+        sincosf(theta_new, &so, &co);
+        return;
+    }
+    const float d2 = d * d;
+    const float sd = d * (1.0f - d2 * (1.0f / 6.0f - d2 * (1.0f / 120.0f)));
+    const float cd = 1.0f - d2 * (0.5f - d2 * (1.0f / 24.0f - d2 * (1.0f / 720.0f)));
+    so = s * cd + c * sd;
+    co = c * cd - s * sd;
+}
+
+// FP32 RK4 step with the reference tableau. (s, c) carry sin/cos of y[2] in
+// and of out[2] out: one accurate sincosf per step, the stage angles θ + δ
+// come from rotations by the small stage increments δ. p^t is re-projected
+// at the end of the step. Returns |sin θ| of the new state (1 for Cartesian
+// charts).
 template <class MF>
-GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out) {
+GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c) {
 #pragma clang fp contract(fast)
     float acc[8], yt[8], dp[4];
-    float s = 0.0f, c = 0.0f;
+    float ss = s, cs = c;
     const float hh = 0.5f * h;
-    if constexpr (MF::uses_theta) sincosf(y[2], &s, &c);
     MF::accel(P, y, dp, s, c);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -243,8 +260,8 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out) {
         yt[i] = y[i] + hh * y[4 + i];
         yt[4 + i] = y[4 + i] + hh * dp[i];
     }
-    if constexpr (MF::uses_theta) sincosf(yt[2], &s, &c);
-    MF::accel(P, yt, dp, s, c);
+    if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], hh * y[6], ss, cs);
+    MF::accel(P, yt, dp, ss, cs);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float kx = yt[4 + i];
@@ -253,8 +270,8 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out) {
         yt[i] = y[i] + hh * kx;
         yt[4 + i] = y[4 + i] + hh * dp[i];
     }
-    if constexpr (MF::uses_theta) sincosf(yt[2], &s, &c);
-    MF::accel(P, yt, dp, s, c);
+    if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], yt[2] - y[2], ss, cs);
+    MF::accel(P, yt, dp, ss, cs);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float kx = yt[4 + i];
@@ -263,8 +280,8 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out) {
         yt[i] = y[i] + h * kx;
         yt[4 + i] = y[4 + i] + h * dp[i];
     }
-    if constexpr (MF::uses_theta) sincosf(yt[2], &s, &c);
-    MF::accel(P, yt, dp, s, c);
+    if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], yt[2] - y[2], ss, cs);
+    MF::accel(P, yt, dp, ss, cs);
     const float sixth = h * (1.0f / 6.0f);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -274,9 +291,9 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
     if constexpr (MF::uses_theta) {
-        const float sn = sinf(out[2]);
-        MF::reproject(P, out, sn);
-        return fabsf(sn);
+        sincosf(out[2], &s, &c);
+        MF::reproject(P, out, s);
+        return fabsf(s);
     } else {
         MF::reproject(P, out);
         return 1.0f;

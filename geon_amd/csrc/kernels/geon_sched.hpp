@@ -131,10 +131,11 @@ struct Prec<MP, double> {
     using Params = geon_dmetric;
     GEON_D static Params params(const geon_dmetric& m) { return m; }
     GEON_D static int terminate(const Params& P, const double* y) { return MP::terminate(P, y); }
-    GEON_D static float step(const Params& P, const double* y, double h, double* out) {
+    GEON_D static float step(const Params& P, const double* y, double h, double* out, float&, float&) {
         rk4_step<MP>(P, y, h, out);
         return 1.0f;
     }
+    GEON_D static void trig(const double*, float&, float&) {}
     GEON_D static float radius(const double*) { return 1e30f; }
     GEON_D static bool finite(const double* y) { return finite8(y); }
     GEON_D static bool suspicious(const Params&, int, const double*, float, float) { return false; }
@@ -146,8 +147,11 @@ struct Prec<MP, float> {
     using Params = FParams;
     GEON_D static Params params(const geon_dmetric& m) { return fparams(m); }
     GEON_D static int terminate(const Params& P, const float* y) { return MF::terminate(P, y); }
-    GEON_D static float step(const Params& P, const float* y, float h, float* out) {
-        return rk4_step_f<MF>(P, y, h, out);
+    GEON_D static float step(const Params& P, const float* y, float h, float* out, float& s, float& c) {
+        return rk4_step_f<MF>(P, y, h, out, s, c);
+    }
+    GEON_D static void trig(const float* y, float& s, float& c) {
+        if constexpr (MF::uses_theta) sincosf(y[2], &s, &c);
     }
     GEON_D static float radius(const float* y) { return MF::radius(y); }
     GEON_D static bool finite(const float* y) { return finite8f(y); }
@@ -171,6 +175,7 @@ __device__ void sched_run(const geon_sched_args& A) {
     T h[RPL];
     int early[RPL];
     float min_s[RPL], min_r[RPL];   // FP32 conditioning trackers
+    float sn[RPL], cs[RPL];         // sin/cos of the current θ (FP32 charts with θ)
     Params P[RPL];
     unsigned long long executed = 0;
 
@@ -187,7 +192,10 @@ __device__ void sched_run(const geon_sched_args& A) {
         min_s[r] = 1.0f;
         min_r[r] = 1e30f;
         act[r] = true;
-        if (!resume) act[r] = init_ray<MP, T>(J, gid, A, y[r]);
+        if (!resume) {
+            act[r] = init_ray<MP, T>(J, gid, A, y[r]);
+            if (act[r]) PR::trig(y[r], sn[r], cs[r]);
+        }
     };
 
     auto finish = [&](int r, int cls) {
@@ -215,6 +223,7 @@ __device__ void sched_run(const geon_sched_args& A) {
                 const T* sy = (const T*)A.slot_y;
 #pragma unroll
                 for (int c = 0; c < 8; ++c) y[r][c] = sy[(size_t)c * A.n_slots + s];
+                PR::trig(y[r], sn[r], cs[r]);
             }
         }
     }
@@ -292,15 +301,18 @@ __device__ void sched_run(const geon_sched_args& A) {
                     continue;
                 }
                 T yn[8];
-                const float sn = PR::step(P[r], y[r], h[r], yn);
+                float s2 = sn[r], c2 = cs[r];
+                const float asn = PR::step(P[r], y[r], h[r], yn, s2, c2);
                 if (!PR::finite(yn)) {
                     finish(r, DIVERGED);
                     continue;
                 }
 #pragma unroll
                 for (int c = 0; c < 8; ++c) y[r][c] = yn[c];
+                sn[r] = s2;
+                cs[r] = c2;
                 if constexpr (std::is_same<T, float>::value) {
-                    min_s[r] = fminf(min_s[r], sn);
+                    min_s[r] = fminf(min_s[r], asn);
                     min_r[r] = fminf(min_r[r], PR::radius(yn));
                 }
                 ++n[r];
