@@ -22,7 +22,7 @@ from ._abi import (ALCUBIERRE, CAPTURED, CURV_EINSTEIN, CURV_RICCI, CURV_RIEMANN
 __all__ = ["Minkowski", "Schwarzschild", "Alcubierre", "MorrisThorne", "Camera", "RenderConfig",
            "Context", "render", "trace_rays", "riemann", "ricci", "ricci_scalar", "einstein",
            "stress_energy", "nec_contraction", "shadow_bisect", "tetrad", "GeonError", "Batch",
-           "autotune"]
+           "autotune", "Server"]
 
 
 # ---- metrics (src/metric/*.rs) ----------------------------------------------
@@ -291,6 +291,81 @@ class Batch:
     def close(self):
         if self.handle:
             lib().geon_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching server (geon_serve_*): a native host thread admits
+    requests by HBM footprint and runs all admitted rays together in quanta."""
+
+    def __init__(self, ctx: Context | None = None, hbm_budget: int = 0, quantum_steps: int = 0,
+                 max_jobs: int = 0):
+        self.ctx = ctx or default_context()
+        cfg = _abi.ServeCfg(hbm_budget=hbm_budget, quantum_steps=quantum_steps, max_jobs=max_jobs)
+        h = C.c_void_p()
+        check(lib().geon_serve_create(self.ctx.handle, C.byref(cfg), C.byref(h)), "geon_serve_create")
+        self.handle = h
+
+    def submit_render(self, metric, camera: Camera, cfg: RenderConfig, keep=False) -> int:
+        t = C.c_uint64()
+        check(lib().geon_serve_submit_render(self.handle, C.byref(_cmetric(metric)),
+                                             C.byref(camera.to_c()), C.byref(cfg.to_c()),
+                                             int(bool(keep)), C.byref(t)), "geon_serve_submit_render")
+        return t.value
+
+    def submit_bisect(self, mass, r_obs, step=0.01, max_steps=200_000, iters=40) -> int:
+        m = np.ascontiguousarray(np.atleast_1d(mass), dtype=np.float64)
+        r = np.ascontiguousarray(np.atleast_1d(r_obs), dtype=np.float64)
+        t = C.c_uint64()
+        check(lib().geon_serve_submit_bisect(self.handle, _ptr(m), _ptr(r), m.size, float(step),
+                                             int(max_steps), int(iters), C.byref(t)),
+              "geon_serve_submit_bisect")
+        return t.value
+
+    def poll(self, ticket: int):
+        """None while pending, else the result dict."""
+        res = _abi.ServeResult()
+        st = lib().geon_serve_poll(self.handle, ticket, C.byref(res))
+        if st == _abi.E_PENDING:
+            return None
+        check(st, "geon_serve_poll")
+        return dict(kind=res.kind, status=res.status, latency_ms=res.latency_ms,
+                    queue_ms=res.queue_ms, stats=res.stats.as_dict(), ray_steps=res.ray_steps)
+
+    def fetch_bisect(self, ticket: int, n: int):
+        a, b = np.zeros(n), np.zeros(n)
+        check(lib().geon_serve_fetch_bisect(self.handle, ticket, _ptr(a), _ptr(b)),
+              "geon_serve_fetch_bisect")
+        return a, b
+
+    def fetch_render(self, ticket: int, width: int, height: int):
+        n = width * height
+        rgb, cls, d = np.zeros((n, 3), np.float32), np.zeros(n, np.int32), np.zeros((3, n))
+        check(lib().geon_serve_fetch_render(self.handle, ticket, _ptr(rgb), _ptr(cls), _ptr(d)),
+              "geon_serve_fetch_render")
+        return rgb, cls, d.T.copy()
+
+    def release(self, ticket: int):
+        check(lib().geon_serve_release(self.handle, ticket), "geon_serve_release")
+
+    def drain(self):
+        check(lib().geon_serve_drain(self.handle), "geon_serve_drain")
+
+    def counters(self) -> dict:
+        q, d, r, b = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_double()
+        check(lib().geon_serve_counters(self.handle, C.byref(q), C.byref(d), C.byref(r),
+                                        C.byref(b)), "geon_serve_counters")
+        return dict(quanta=q.value, done=d.value, ray_steps=r.value, busy_ms=b.value)
+
+    def close(self):
+        if self.handle:
+            lib().geon_serve_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -53,6 +53,11 @@ class SchedCfg(C.Structure):
                 ("rays_per_lane", C.c_uint32), ("refill", C.c_uint32)]
 
 
+class ServeCfg(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("quantum_steps", C.c_uint32),
+                ("max_jobs", C.c_uint32)]
+
+
 class RenderStats(C.Structure):
     _fields_ = [("captured", C.c_uint64), ("escaped", C.c_uint64), ("diverged", C.c_uint64),
                 ("max_null_residual", C.c_double), ("avg_steps", C.c_double),
@@ -61,6 +66,11 @@ class RenderStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("status", C.c_int32), ("latency_ms", C.c_double),
+                ("queue_ms", C.c_double), ("stats", RenderStats), ("ray_steps", C.c_uint64)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/geon_hip.h
@@ -97,6 +107,19 @@ SIGNATURES = [
     ("geon_batch_totals", _i, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
                                C.POINTER(C.c_double)]),
     ("geon_autotune", _i, [_vp, C.c_int32, C.c_int32, _vp]),
+    ("geon_serve_create", _i, [_vp, C.POINTER(ServeCfg), C.POINTER(_vp)]),
+    ("geon_serve_destroy", None, [_vp]),
+    ("geon_serve_submit_render", _i, [_vp, C.POINTER(Metric), C.POINTER(Camera),
+                                      C.POINTER(RenderCfg), C.c_int32, C.POINTER(C.c_uint64)]),
+    ("geon_serve_submit_bisect", _i, [_vp, _vp, _vp, _sz, C.c_double, _u32, _u32,
+                                      C.POINTER(C.c_uint64)]),
+    ("geon_serve_poll", _i, [_vp, C.c_uint64, C.POINTER(ServeResult)]),
+    ("geon_serve_fetch_bisect", _i, [_vp, C.c_uint64, _vp, _vp]),
+    ("geon_serve_fetch_render", _i, [_vp, C.c_uint64, _vp, _vp, _vp]),
+    ("geon_serve_release", _i, [_vp, C.c_uint64]),
+    ("geon_serve_drain", _i, [_vp]),
+    ("geon_serve_counters", _i, [_vp, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
 ]
 
 _lib = None

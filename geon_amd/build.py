@@ -32,7 +32,7 @@ KERNEL_DEPS = KERNEL_SRC + [os.path.join(CSRC, "kernels", f)
                             for f in ("geon_device.hpp", "geon_fp32.hpp", "geon_sched.hpp")] + \
     [os.path.join(CSRC, "geon_dev_types.h")]
 HOST_SRC = [os.path.join(CSRC, "host", f)
-            for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp")]
+            for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp", "geon_serve.cpp")]
 
 
 def _run(cmd):
@@ -54,7 +54,8 @@ def build_host(out=None):
     os.makedirs(os.path.dirname(out), exist_ok=True)
     tmp = out + ".tmp"
     cxx = shutil.which("g++") or "g++"
-    _run([cxx, *HOST_FLAGS, *HOST_SRC, f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib",
+    _run([cxx, *HOST_FLAGS, *HOST_SRC, f"-L{ROCM}/lib", "-lamdhip64", "-lpthread",
+          f"-Wl,-rpath,{ROCM}/lib",
           "-o", tmp])
     os.replace(tmp, out)
     return out

@@ -88,13 +88,14 @@ struct geon_sched_args {
     unsigned long long end;               // claimable positions are [cursor, end)
     const unsigned long long* end_ptr;    // if set, end = *end_ptr at kernel start
     const unsigned long long* list;       // list mode: position → ray id (else position = id)
+    unsigned long long list_cap;          // list mode: ring capacity (position taken modulo)
     uint32_t quantum;                     // step budget per lane (0 = unbounded)
     uint32_t n_slots;                     // lanes × rays-per-lane
     void* slot_y;                         // SoA 8 × n_slots of T (quantum mode)
     unsigned long long* slot_g;           // ray id per slot, ~0 = empty
     uint32_t* slot_n;                     // steps taken per slot
-    unsigned long long* fb_list;          // FP64 fallback ray ids (FP32 kernels)
-    unsigned long long* fb_count;
-    unsigned long long fb_cap;
+    unsigned long long* fb_list;          // FP64 fallback ray ids (FP32 kernels), a ring
+    unsigned long long* fb_count;         // entries ever pushed
+    unsigned long long fb_cap;            // ring capacity
     unsigned long long* ray_steps;        // Σ executed RK4 steps (for throughput)
 };

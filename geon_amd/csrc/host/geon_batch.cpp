@@ -204,6 +204,7 @@ int batch_run_on(geon_batch* b, const geon_sched_cfg_t* tune, hipStream_t user) 
             f.cursor = ck + C_FBCURSOR;
             f.end_ptr = ck + C_FBCOUNT;
             f.list = b->d_fb_list[k];
+            f.list_cap = b->fb_cap[k];
             f.ray_steps = ck + C_STEPSFB;
             // the list length is only known on the device: size for the full GPU
             GEON_TRY(sched_launch(c, k, 0, cf, f, 0, s));

@@ -198,6 +198,8 @@ int ctx_create(int device, const char* code_object, geon_ctx** out) {
     ok = ok && get(&c->fn_bisect, "geon_bisect_serial");
     ok = ok && get(&c->fn_job_finalize, "geon_job_finalize");
     ok = ok && get(&c->fn_zero, "geon_zero_u64");
+    ok = ok && get(&c->fn_fill, "geon_fill_u64");
+    ok = ok && get(&c->fn_clamp, "geon_clamp_cursor");
     for (int k = 0; k < 4 && ok; ++k)
         for (int f = 0; f < 2 && ok; ++f)
             for (int r = 0; r < 2 && ok; ++r)

@@ -20,6 +20,8 @@ struct geon_ctx {
     hipFunction_t fn_sched[4][2][2] = {};   // [kind][0 f64 | 1 f32][rays_per_lane - 1]
     hipFunction_t fn_job_finalize = nullptr;
     hipFunction_t fn_zero = nullptr;
+    hipFunction_t fn_fill = nullptr;
+    hipFunction_t fn_clamp = nullptr;
     int n_cu = 256;
     geon_sched_cfg_t tuned[4][2] = {};       // [kind][0 FP64 | 1 FP32-mixed]
     hipStream_t kind_stream[4] = {};

@@ -205,6 +205,23 @@ extern "C" __global__ void geon_zero_u64(unsigned long long* p, uint64_t n) {
         p[i] = 0ull;
 }
 
+// Fill n 64-bit words with v (empty slot markers).
+extern "C" __global__ void geon_fill_u64(unsigned long long* p, uint64_t n, unsigned long long v) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// Between serving quanta: a queue cursor that ran past the claimable end is
+// pulled back to it (bound read from `bound_ptr` when given, else `bound`).
+extern "C" __global__ void geon_clamp_cursor(unsigned long long* cursor, unsigned long long bound,
+                                             const unsigned long long* bound_ptr) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const unsigned long long b = bound_ptr ? *bound_ptr : bound;
+        if (*cursor > b) *cursor = b;
+    }
+}
+
 // Zero a statistics block (used instead of a memset node so the sequence
 // stays valid inside captured graphs).
 extern "C" __global__ void geon_stats_clear(geon_dstats* stats) {

@@ -13,6 +13,8 @@
 // Quantum mode (serving): a step budget bounds each launch; rays still in
 // flight are written to a slot pool and resumed by the next launch, so
 // requests admitted between launches join the rays already running.
+// The FP64 hand-back list is a ring: entries pushed but not yet claimed
+// never exceed the rays of the jobs admitted, which bounds its capacity.
 #pragma once
 #include "geon_fp32.hpp"
 
@@ -105,7 +107,7 @@ GEON_D bool init_ray(const geon_djob& J, unsigned long long gid,
     if constexpr (std::is_same<T, float>::value) {
         if (J.F.precision == 1 && near_photon_orbit<MP>(J.F, yd)) {
             const unsigned long long q = atomicAdd(A.fb_count, 1ull);
-            if (q < A.fb_cap) A.fb_list[q] = gid;
+            A.fb_list[q % A.fb_cap] = gid;
             if (J.fp64) atomicAdd(J.fp64, 1ull);
             return false;
         }
@@ -118,7 +120,7 @@ GEON_D bool init_ray(const geon_djob& J, unsigned long long gid,
 template <class T>
 GEON_D void push_fallback(const geon_sched_args& A, const geon_djob& J, unsigned long long gid) {
     const unsigned long long q = atomicAdd(A.fb_count, 1ull);
-    if (q < A.fb_cap) A.fb_list[q] = gid;
+    A.fb_list[q % A.fb_cap] = gid;
     if (J.fp64) atomicAdd(J.fp64, 1ull);
 }
 
@@ -261,7 +263,7 @@ __device__ void sched_run(const geon_sched_args& A) {
                     const uint32_t off = before + (uint32_t)__popcll(below);
                     if (!act[r] && off < take) {
                         const unsigned long long pos = base + off;
-                        activate(r, A.list ? A.list[pos] : pos, 0, false);
+                        activate(r, A.list ? A.list[pos % A.list_cap] : pos, 0, false);
                     }
                     before += __popcll(masks[r]);
                 }

@@ -250,6 +250,61 @@ int  geon_batch_totals(geon_batch_t* batch, uint64_t* ray_steps, uint64_t* fp64_
 int  geon_autotune(geon_ctx_t* ctx, int32_t metric_kind, int32_t precision,
                    geon_sched_cfg_t* best);
 
+/* ---- serving (no reference counterpart: the reference renders one frame per
+ * process, bin/render.rs, and bisects serially, bin/validate_shadow.rs) ------
+ * A server owns a host thread that admits requests as HBM frees up and runs
+ * every admitted request's rays together in step-bounded quanta of the
+ * persistent scheduler: requests that arrive while others are on the GPU join
+ * the rays already in flight. */
+typedef struct geon_serve_cfg {
+    uint64_t hbm_budget;       /* bytes of request buffers in flight (0: 8 GiB)       */
+    uint32_t quantum_steps;    /* RK4 steps per lane per quantum (0: 256)             */
+    uint32_t max_jobs;         /* job-table capacity over the server's life (0: 16384) */
+} geon_serve_cfg_t;
+
+#define GEON_REQ_RENDER  0
+#define GEON_REQ_BISECT  1
+
+typedef struct geon_serve_result {
+    int32_t  kind;             /* GEON_REQ_*                                           */
+    int32_t  status;           /* GEON_OK, or the error that ended the request         */
+    double   latency_ms;       /* submit → result available                            */
+    double   queue_ms;         /* submit → admitted                                    */
+    geon_render_stats_t stats; /* render requests                                      */
+    uint64_t ray_steps;        /* Σ steps_taken of the request's final rays           */
+} geon_serve_result_t;
+
+typedef struct geon_server geon_server_t;
+
+int  geon_serve_create(geon_ctx_t* ctx, const geon_serve_cfg_t* cfg, geon_server_t** out);
+/* Stops the host thread after the requests in flight finish, then frees. */
+void geon_serve_destroy(geon_server_t* srv);
+/* Render request; outputs are kept by the server when keep_outputs != 0 and
+ * can be copied out with geon_serve_fetch_render before geon_serve_release. */
+int  geon_serve_submit_render(geon_server_t* srv, const geon_metric_t* metric,
+                              const geon_camera_t* cam, const geon_render_cfg_t* cfg,
+                              int32_t keep_outputs, uint64_t* ticket);
+/* Shadow bisection request: n problems (mass[k], r_obs[k]), `iters` halvings
+ * each with the capture predicate trace(step, max_steps) on equatorial rays
+ * (validate_shadow.rs:20-79), run as parallel k-section rounds. */
+int  geon_serve_submit_bisect(geon_server_t* srv, const double* mass, const double* r_obs,
+                              size_t n, double step, uint32_t max_steps, uint32_t iters,
+                              uint64_t* ticket);
+/* GEON_OK with the result when done, GEON_E_PENDING while running. */
+int  geon_serve_poll(geon_server_t* srv, uint64_t ticket, geon_serve_result_t* out);
+/* Bisection results (alpha_c, b_c: n doubles each, host). */
+int  geon_serve_fetch_bisect(geon_server_t* srv, uint64_t ticket, double* alpha_c, double* b_c);
+/* Render outputs (host buffers, any may be NULL; layout as geon_render). */
+int  geon_serve_fetch_render(geon_server_t* srv, uint64_t ticket, float* rgb,
+                             int32_t* cls_per_pixel, double* dir_per_pixel);
+int  geon_serve_release(geon_server_t* srv, uint64_t ticket);
+/* Blocks until every submitted request is done. */
+int  geon_serve_drain(geon_server_t* srv);
+/* Server-side counters: quanta launched, requests done, Σ final ray-steps,
+ * device-busy milliseconds. */
+int  geon_serve_counters(geon_server_t* srv, uint64_t* quanta, uint64_t* done,
+                         uint64_t* ray_steps, double* busy_ms);
+
 /* write_png's per-channel map: clamp(0,1)^(1/2.2)·255, rounded (render.rs:175-179). */
 int  geon_tonemap_u8(const float* rgb, size_t n, uint8_t* out);
 
