@@ -93,6 +93,7 @@ struct Queue {
     void* slot_y = nullptr;
     unsigned long long* slot_g = nullptr;
     uint32_t* slot_n = nullptr;
+    float* slot_aux = nullptr;
     int outstanding = 0;       // jobs not yet complete
 };
 
@@ -193,6 +194,7 @@ int init_queue(geon_server* s, Queue& Q, int kind, int prec, uint32_t cap_jobs) 
     GEON_HIP(hipMalloc(&Q.slot_y, (size_t)Q.n_slots * 8 * tsz), "slots");
     GEON_HIP(hipMalloc((void**)&Q.slot_g, (size_t)Q.n_slots * 8), "slots");
     GEON_HIP(hipMalloc((void**)&Q.slot_n, (size_t)Q.n_slots * 4), "slots");
+    GEON_HIP(hipMalloc((void**)&Q.slot_aux, (size_t)Q.n_slots * 8), "slots");
     uint64_t n = Q.n_slots;
     unsigned long long empty = ~0ull;
     void* a[] = {&Q.slot_g, &n, &empty};
@@ -206,6 +208,7 @@ void free_queue(Queue& Q) {
     if (Q.slot_y) hipFree(Q.slot_y);
     if (Q.slot_g) hipFree(Q.slot_g);
     if (Q.slot_n) hipFree(Q.slot_n);
+    if (Q.slot_aux) hipFree(Q.slot_aux);
 }
 
 // Append a job to queue (kind, prec); returns the global job index.
@@ -379,6 +382,7 @@ int launch_quantum(geon_server* s, Queue& Q, int kind, int f32, unsigned long lo
     a.slot_y = Q.slot_y;
     a.slot_g = Q.slot_g;
     a.slot_n = Q.slot_n;
+    a.slot_aux = Q.slot_aux;
     a.ray_steps = s->d_ctr + (size_t)kind * K_WORDS + K_STEPS;
     if (f32) {
         a.fb_list = s->d_list[kind];

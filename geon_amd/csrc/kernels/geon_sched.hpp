@@ -226,6 +226,10 @@ __device__ void sched_run(const geon_sched_args& A) {
 #pragma unroll
                 for (int c = 0; c < 8; ++c) y[r][c] = sy[(size_t)c * A.n_slots + s];
                 PR::trig(y[r], sn[r], cs[r]);
+                if (A.slot_aux) {
+                    min_s[r] = A.slot_aux[s];
+                    min_r[r] = A.slot_aux[A.n_slots + s];
+                }
             }
         }
     }
@@ -336,6 +340,10 @@ __device__ void sched_run(const geon_sched_args& A) {
                 T* sy = (T*)A.slot_y;
 #pragma unroll
                 for (int c = 0; c < 8; ++c) sy[(size_t)c * A.n_slots + s] = y[r][c];
+                if (A.slot_aux) {
+                    A.slot_aux[s] = min_s[r];
+                    A.slot_aux[A.n_slots + s] = min_r[r];
+                }
             } else {
                 A.slot_g[s] = ~0ull;
             }
