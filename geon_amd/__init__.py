@@ -319,12 +319,14 @@ class Server:
                                              int(bool(keep)), C.byref(t)), "geon_serve_submit_render")
         return t.value
 
-    def submit_bisect(self, mass, r_obs, step=0.01, max_steps=200_000, iters=40) -> int:
+    def submit_bisect(self, mass, r_obs, step=0.01, max_steps=200_000, iters=40,
+                      full_trace=False) -> int:
         m = np.ascontiguousarray(np.atleast_1d(mass), dtype=np.float64)
         r = np.ascontiguousarray(np.atleast_1d(r_obs), dtype=np.float64)
         t = C.c_uint64()
         check(lib().geon_serve_submit_bisect(self.handle, _ptr(m), _ptr(r), m.size, float(step),
-                                             int(max_steps), int(iters), C.byref(t)),
+                                             int(max_steps), int(iters), int(bool(full_trace)),
+                                             C.byref(t)),
               "geon_serve_submit_bisect")
         return t.value
 

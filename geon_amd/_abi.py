@@ -111,7 +111,7 @@ SIGNATURES = [
     ("geon_serve_destroy", None, [_vp]),
     ("geon_serve_submit_render", _i, [_vp, C.POINTER(Metric), C.POINTER(Camera),
                                       C.POINTER(RenderCfg), C.c_int32, C.POINTER(C.c_uint64)]),
-    ("geon_serve_submit_bisect", _i, [_vp, _vp, _vp, _sz, C.c_double, _u32, _u32,
+    ("geon_serve_submit_bisect", _i, [_vp, _vp, _vp, _sz, C.c_double, _u32, _u32, C.c_int32,
                                       C.POINTER(C.c_uint64)]),
     ("geon_serve_poll", _i, [_vp, C.c_uint64, C.POINTER(ServeResult)]),
     ("geon_serve_fetch_bisect", _i, [_vp, C.c_uint64, _vp, _vp]),

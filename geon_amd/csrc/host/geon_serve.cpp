@@ -117,6 +117,7 @@ struct Request {
     std::vector<int32_t> round_cls;
     double step = 0.01;
     uint32_t max_steps = 0, iters = 0, done_iters = 0, levels = 0;
+    bool full_trace = false;
     double* d_y0 = nullptr;
     int32_t* d_cls = nullptr;
     uint32_t* d_steps = nullptr;
@@ -295,7 +296,7 @@ int start_round(geon_server* s, Request& r) {
     GEON_TRY(frame_from(&m, nullptr, &cfg, &dj.F));
     dj.n_rays = nr;
     dj.gen = 1;
-    dj.early_exit = 1;
+    dj.early_exit = r.full_trace ? 0 : 1;
     dj.y0 = r.d_y0;
     dj.cls = r.d_cls;
     dj.steps = r.d_steps;
@@ -698,7 +699,8 @@ int geon_serve_submit_render(geon_server_t* s, const geon_metric_t* metric, cons
 }
 
 int geon_serve_submit_bisect(geon_server_t* s, const double* mass, const double* r_obs, size_t n,
-                             double step, uint32_t max_steps, uint32_t iters, uint64_t* ticket) {
+                             double step, uint32_t max_steps, uint32_t iters, int32_t full_trace,
+                             uint64_t* ticket) {
     if (!s || !mass || !r_obs || n == 0 || iters == 0) return fail(GEON_E_INVALID, "bad argument");
     for (size_t k = 0; k < n; ++k) {
         if (!(mass[k] > 0.0) || !(r_obs[k] > 2.0 * mass[k]))
@@ -713,6 +715,7 @@ int geon_serve_submit_bisect(geon_server_t* s, const double* mass, const double*
     r->step = step;
     r->max_steps = max_steps;
     r->iters = iters;
+    r->full_trace = full_trace != 0;
     return submit(s, r, ticket);
 }
 

@@ -284,12 +284,17 @@ void geon_serve_destroy(geon_server_t* srv);
 int  geon_serve_submit_render(geon_server_t* srv, const geon_metric_t* metric,
                               const geon_camera_t* cam, const geon_render_cfg_t* cfg,
                               int32_t keep_outputs, uint64_t* ticket);
-/* Shadow bisection request: n problems (mass[k], r_obs[k]), `iters` halvings
- * each with the capture predicate trace(step, max_steps) on equatorial rays
- * (validate_shadow.rs:20-79), run as parallel k-section rounds. */
+/* Shadow bisection request: n problems (mass[k], r_obs[k]) of one mass,
+ * `iters` halvings each with the capture predicate trace(step, max_steps) on
+ * equatorial rays (validate_shadow.rs:20-79), run as parallel k-section
+ * rounds that visit exactly the serial bisection's midpoints.
+ * full_trace = 0 decides each probe at the photon sphere (outgoing beyond
+ * 3M: escaped, ingoing inside: captured), which converges on 3√3 M; with
+ * full_trace = 1 every probe is traced to capture or escape as the reference
+ * does, reproducing its fixed-step result. */
 int  geon_serve_submit_bisect(geon_server_t* srv, const double* mass, const double* r_obs,
                               size_t n, double step, uint32_t max_steps, uint32_t iters,
-                              uint64_t* ticket);
+                              int32_t full_trace, uint64_t* ticket);
 /* GEON_OK with the result when done, GEON_E_PENDING while running. */
 int  geon_serve_poll(geon_server_t* srv, uint64_t ticket, geon_serve_result_t* out);
 /* Bisection results (alpha_c, b_c: n doubles each, host). */

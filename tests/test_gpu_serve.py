@@ -98,17 +98,20 @@ def test_serve_bisection(ctx, O):
     srv = geon.Server(ctx, hbm_budget=1 << 28, quantum_steps=512)
     r_obs = np.linspace(10.0, 50.0, 64)
     t = srv.submit_bisect(np.ones(64), r_obs, step=0.01, max_steps=200_000, iters=40)
-    t2 = srv.submit_bisect([1.0], [20.0], step=0.01, max_steps=200_000, iters=60)
+    t2 = srv.submit_bisect([1.0], [20.0], step=0.01, max_steps=200_000, iters=60,
+                           full_trace=True)
     res = _wait(srv, [t, t2], timeout=300.0)
     assert res[t]["status"] == 0 and res[t2]["status"] == 0
     a, b = srv.fetch_bisect(t, 64)
-    # 40 halvings of [0, π/2 − 1e-3] leave a bracket of 1.4e-12 rad
-    assert np.abs(b - 3 * math.sqrt(3)).max() < 1e-6
+    # photon-sphere decisions converge on the analytic b_c = 3√3 M
+    assert np.abs(b / (3 * math.sqrt(3)) - 1.0).max() < 1e-6
+    # full traces reproduce validate_shadow (README.md:43-45) through the k-section rounds
     a2, b2 = srv.fetch_bisect(t2, 1)
     a_ref, b_ref = O.validate_shadow(1.0, 20.0, 0.01, 200_000, 60)
     assert abs(b2[0] - b_ref) < 1e-9
     assert f"{b2[0]:.6f}" == "5.196151"
-    print(f"bisection: 64×40 latency {res[t]['latency_ms']:.1f} ms, ray-steps {res[t]['ray_steps']}")
+    print(f"bisection: 64×40 latency {res[t]['latency_ms']:.1f} ms, ray-steps {res[t]['ray_steps']}; "
+          f"full-trace 1×60 latency {res[t2]['latency_ms']:.1f} ms")
     srv.close()
 
 
