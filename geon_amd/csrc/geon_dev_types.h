@@ -99,4 +99,5 @@ struct geon_sched_args {
     unsigned long long* fb_count;         // entries ever pushed
     unsigned long long fb_cap;            // ring capacity
     unsigned long long* ray_steps;        // Σ executed RK4 steps (for throughput)
+    unsigned long long* parked;           // quantum mode: rays left in flight (added to)
 };

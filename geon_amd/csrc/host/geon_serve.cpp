@@ -1,16 +1,23 @@
 // geon_serve.cpp — continuous-batching server over the persistent scheduler.
 //
-// One host thread owns the GPU work. Each loop iteration it
+// One host thread owns the GPU work. Each metric kind has a pipeline on its
+// own stream; the pipelines run independently of each other. The loop
 //   1. admits queued requests in arrival order while their buffers fit in the
 //      HBM arena (first-fit sub-allocation of one device allocation);
-//   2. launches one step-bounded quantum per active queue: per metric kind an
-//      FP32 queue, its FP64 fallback list and an FP64 queue, kinds on their
-//      own streams. Rays still running at the end of a quantum are parked in
-//      the queue's slot pool and resume in the next quantum, so rays of newly
-//      admitted requests start next to rays already in flight;
-//   3. reads back the per-job completion counters and harvests finished
-//      jobs: renders get their per-pixel reduction, bisection requests walk
-//      their k-section trees and enqueue the next round of probes.
+//   2. for every idle pipeline with work, enqueues one step-bounded quantum
+//      of each of its active queues (FP32, its FP64 hand-back list, FP64),
+//      then an asynchronous read-back of its counters and of its jobs'
+//      completion words, and an event. Rays still running at the end of a
+//      quantum are parked in the queue's slot pool and resume in the next
+//      quantum, so rays of newly admitted requests start next to rays
+//      already in flight;
+//   3. polls the events; when a pipeline's quantum ends it harvests that
+//      kind's finished jobs: renders get their per-pixel reduction,
+//      bisection requests walk their k-section trees and enqueue the next
+//      round of probes.
+// The FP64 hand-back list of a kind is drained while FP32 work no longer
+// fills the GPU or once it holds a quarter of the FP64 slots, so FP64
+// quanta run at useful occupancy under load.
 // Per-ray arithmetic never depends on the slot, lane or neighbours of a ray,
 // so a request's result is the same alone and in any mix.
 #include <math.h>
@@ -97,6 +104,19 @@ struct Queue {
     int outstanding = 0;       // jobs not yet complete
 };
 
+// A metric kind's quantum pipeline: its stream, the event that ends the
+// last enqueued quantum and pinned read-back buffers.
+struct Pipe {
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    bool busy = false;
+    unsigned long long* h_ctr = nullptr;  // K_WORDS counters of the kind
+    unsigned long long* h_job = nullptr;  // completion words of jobs [lo, hi]
+    int lo = 0, hi = -1;
+    bool ran32 = false, ranfb = false;    // queues run by the quantum in flight
+    unsigned long long parked32 = 0, parkedfb = 0;  // rays parked by their last quanta
+};
+
 struct Request {
     uint64_t ticket = 0;
     int kind = GEON_REQ_RENDER;
@@ -137,18 +157,15 @@ struct geon_server {
     geon_serve_cfg_t cfg;
     Arena arena;
     void* arena_mem = nullptr;
-    Queue q[4][2];                       // [kind][0 FP64 | 1 FP32]
+    Queue q[4][3];                       // [kind][0 FP64 | 1 FP32 | 2 FP64 probes, one per lane]
     Queue fb[4];                         // FP64 pass over FP32 hand-backs
     unsigned long long* d_list[4] = {};  // fallback ray ids
     uint64_t fb_cap = 0;
-    unsigned long long* d_ctr = nullptr; // per kind: cursor64, cursor32, fbcursor, fbcount, steps
+    unsigned long long* d_ctr = nullptr; // per kind: K_WORDS counters
     unsigned long long* d_job = nullptr; // per job: kJobWords
-    std::vector<unsigned long long> h_job;
     uint32_t n_jobs_total = 0;
     hipStream_t stream = nullptr;
-    hipStream_t kind_stream[4] = {};
-    hipEvent_t fork_ev = nullptr;
-    hipEvent_t join_ev[4] = {};
+    Pipe pipe[4];                        // one per metric kind
     // host state
     std::mutex mu;
     std::condition_variable cv, done_cv;
@@ -167,7 +184,10 @@ struct geon_server {
 
 namespace {
 
-enum { K_CUR64, K_CUR32, K_FBCUR, K_FBCOUNT, K_STEPS, K_WORDS };
+// per-kind device counters; K_PARK* count the rays each queue left parked
+// in its last quantum (zeroed before every quantum)
+enum { K_CUR64, K_CUR32, K_FBCUR, K_FBCOUNT, K_STEPS, K_CURB, K_PARK32, K_PARKFB, K_PARK64, K_PARKB,
+       K_WORDS };
 
 int alloc_req(geon_server* s, Request& r, size_t bytes, void** p) {
     size_t off;
@@ -182,10 +202,10 @@ void free_req(geon_server* s, Request& r) {
     r.ranges.clear();
 }
 
-int init_queue(geon_server* s, Queue& Q, int kind, int prec, uint32_t cap_jobs) {
+int init_queue(geon_server* s, Queue& Q, int kind, int prec, uint32_t cap_jobs, bool one_per_lane) {
     geon_ctx* c = s->ctx;
     Q.cfg = c->tuned[kind][prec];
-    if (Q.cfg.rays_per_lane != 2) Q.cfg.rays_per_lane = 1;
+    if (Q.cfg.rays_per_lane != 2 || one_per_lane) Q.cfg.rays_per_lane = 1;
     Q.lanes = (uint32_t)(c->n_cu * 4u * Q.cfg.waves_per_simd * 64u);
     Q.lanes = (Q.lanes + Q.cfg.block - 1) / Q.cfg.block * Q.cfg.block;
     Q.n_slots = Q.lanes * Q.cfg.rays_per_lane;
@@ -302,8 +322,8 @@ int start_round(geon_server* s, Request& r) {
     dj.steps = r.d_steps;
     r.n_rays = nr;
     r.mk = GEON_METRIC_SCHWARZSCHILD;
-    r.mq = 0;
-    GEON_TRY(add_job(s, r.mk, 0, dj, &r.job));
+    r.mq = 2;    // latency-bound chains of rounds: one ray per lane
+    GEON_TRY(add_job(s, r.mk, r.mq, dj, &r.job));
     r.qpos = s->last_qpos;
     return GEON_OK;
 }
@@ -365,7 +385,7 @@ size_t footprint(const Request& r) {
 int launch_quantum(geon_server* s, Queue& Q, int kind, int f32, unsigned long long* cursor,
                    unsigned long long end, const unsigned long long* end_ptr,
                    const unsigned long long* list, hipStream_t st, const geon_djob* jobs,
-                   const unsigned long long* base, uint32_t n_jobs) {
+                   const unsigned long long* base, uint32_t n_jobs, unsigned long long* parked) {
     geon_ctx* c = s->ctx;
     geon_sched_args a;
     memset(&a, 0, sizeof a);
@@ -385,6 +405,7 @@ int launch_quantum(geon_server* s, Queue& Q, int kind, int f32, unsigned long lo
     a.slot_n = Q.slot_n;
     a.slot_aux = Q.slot_aux;
     a.ray_steps = s->d_ctr + (size_t)kind * K_WORDS + K_STEPS;
+    a.parked = parked;
     if (f32) {
         a.fb_list = s->d_list[kind];
         a.fb_count = s->d_ctr + (size_t)kind * K_WORDS + K_FBCOUNT;
@@ -400,7 +421,7 @@ int launch_quantum(geon_server* s, Queue& Q, int kind, int f32, unsigned long lo
     return GEON_OK;
 }
 
-int finish_render(geon_server* s, Request& r) {
+int finish_render(geon_server* s, Request& r, hipStream_t st) {
     geon_ctx* c = s->ctx;
     Queue& Q = s->q[r.mk][r.mq];
     // job index within its queue: jobs are appended in order, find it by base
@@ -408,11 +429,11 @@ int finish_render(geon_server* s, Request& r) {
     void* fargs[] = {&dptr};
     const uint64_t n_pix = (uint64_t)r.cam.width * r.cam.height;
     GEON_HIP(hipModuleLaunchKernel(c->fn_job_finalize, (uint32_t)((n_pix + 255) / 256), 1, 1, 256, 1,
-                                   1, 0, s->stream, fargs, nullptr), "finalize");
+                                   1, 0, st, fargs, nullptr), "finalize");
     unsigned long long w[kJobWords];
     GEON_HIP(hipMemcpyDtoHAsync(w, (hipDeviceptr_t)(s->d_job + (size_t)r.job * kJobWords), sizeof w,
-                                s->stream), "stats");
-    GEON_HIP(hipStreamSynchronize(s->stream), "sync");
+                                st), "stats");
+    GEON_HIP(hipStreamSynchronize(st), "sync");
     geon_dstats ds;
     memcpy(&ds, w + 2, sizeof ds);
     geon_render_stats_t& S = r.result.stats;
@@ -471,14 +492,108 @@ int finish_round(geon_server* s, Request& r, bool* more) {
     return GEON_OK;
 }
 
+// Enqueue one quantum of every active queue of kind k, then the read-back
+// of the kind's counters and of its running jobs' completion words.
+int launch_kind(geon_server* s, int k) {
+    geon_ctx* c = s->ctx;
+    Pipe& P = s->pipe[k];
+    Queue& Q64 = s->q[k][0];
+    Queue& Q32 = s->q[k][1];
+    Queue& FB = s->fb[k];
+    unsigned long long* ck = s->d_ctr + (size_t)k * K_WORDS;
+    const hipStream_t st = P.stream;
+    unsigned long long* park = ck + K_PARK32;
+    uint64_t n_park = 4;
+    void* za[] = {&park, &n_park};
+    GEON_HIP(hipModuleLaunchKernel(c->fn_zero, 1, 1, 1, 64, 1, 1, 0, st, za, nullptr), "zero");
+    if (Q32.outstanding) {
+        GEON_TRY(launch_quantum(s, Q32, k, 1, ck + K_CUR32, Q32.end, nullptr, nullptr, st, Q32.d_jobs,
+                                Q32.d_base, Q32.n_jobs, ck + K_PARK32));
+        // hand-backs known at the last read-back (this quantum may add more)
+        const unsigned long long* h = P.h_ctr;
+        const unsigned long long fb_wait = (h[K_FBCOUNT] - h[K_FBCUR]) + P.parkedfb;
+        const unsigned long long f32_left = (Q32.end > h[K_CUR32] ? Q32.end - h[K_CUR32] : 0) + P.parked32;
+        P.ranfb = fb_wait > 0 && (f32_left < Q32.n_slots || 4 * fb_wait >= FB.n_slots);
+        if (P.ranfb)
+            GEON_TRY(launch_quantum(s, FB, k, 0, ck + K_FBCUR, 0, ck + K_FBCOUNT, s->d_list[k], st,
+                                    Q32.d_jobs, Q32.d_base, Q32.n_jobs, ck + K_PARKFB));
+    } else {
+        P.ranfb = false;
+    }
+    P.ran32 = Q32.outstanding > 0;
+    if (Q64.outstanding)
+        GEON_TRY(launch_quantum(s, Q64, k, 0, ck + K_CUR64, Q64.end, nullptr, nullptr, st, Q64.d_jobs,
+                                Q64.d_base, Q64.n_jobs, ck + K_PARK64));
+    Queue& QB = s->q[k][2];
+    if (QB.outstanding)
+        GEON_TRY(launch_quantum(s, QB, k, 0, ck + K_CURB, QB.end, nullptr, nullptr, st, QB.d_jobs,
+                                QB.d_base, QB.n_jobs, ck + K_PARKB));
+    GEON_HIP(hipMemcpyDtoHAsync(P.h_ctr, (hipDeviceptr_t)ck, K_WORDS * 8, st), "counters");
+    int lo = 1 << 30, hi = -1;
+    for (auto& r : s->running) {
+        if (r->mk != k) continue;
+        lo = r->job < lo ? r->job : lo;
+        hi = r->job > hi ? r->job : hi;
+    }
+    P.lo = lo;
+    P.hi = hi;
+    if (hi >= lo)
+        GEON_HIP(hipMemcpyDtoHAsync(P.h_job, (hipDeviceptr_t)(s->d_job + (size_t)lo * kJobWords),
+                                    (size_t)(hi - lo + 1) * kJobWords * 8, st), "job words");
+    GEON_HIP(hipEventRecord(P.done, st), "event");
+    P.busy = true;
+    s->quanta++;
+    return GEON_OK;
+}
+
+// After kind k's quantum: finish the jobs whose rays are all done.
+void harvest_kind(geon_server* s, int k) {
+    Pipe& P = s->pipe[k];
+    std::vector<std::shared_ptr<Request>> still;
+    for (auto& r : s->running) {
+        if (r->mk != k || r->job < P.lo || r->job > P.hi ||
+            P.h_job[(size_t)(r->job - P.lo) * kJobWords] < r->n_rays) {
+            still.push_back(r);
+            continue;
+        }
+        s->q[r->mk][r->mq].outstanding--;
+        int st = GEON_OK;
+        bool more = false;
+        if (r->kind == GEON_REQ_RENDER) {
+            st = finish_render(s, *r, P.stream);
+        } else {
+            st = finish_round(s, *r, &more);
+        }
+        if (more && st == GEON_OK) {
+            still.push_back(r);
+            continue;
+        }
+        std::lock_guard<std::mutex> lk(s->mu);
+        r->result.status = st;
+        r->result.ray_steps = r->ray_steps;
+        r->t_done = now_ms();
+        r->result.latency_ms = r->t_done - r->t_submit;
+        r->result.queue_ms = r->t_admit - r->t_submit;
+        r->state = 2;
+        if (!r->keep || r->kind == GEON_REQ_BISECT) free_req(s, *r);
+        s->n_done++;
+        s->steps_done += r->ray_steps;
+        s->done_cv.notify_all();
+    }
+    s->running.swap(still);
+}
+
 void serve_loop(geon_server* s) {
     geon_ctx* c = s->ctx;
     hipSetDevice(c->device);
+    double busy_since = -1.0;
     for (;;) {
+        bool any_busy = false;
+        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy;
         // ---- admission ----
         {
             std::unique_lock<std::mutex> lk(s->mu);
-            if (s->running.empty() && s->pending.empty()) {
+            if (s->running.empty() && s->pending.empty() && !any_busy) {
                 if (s->stop) break;
                 s->cv.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
@@ -512,75 +627,38 @@ void serve_loop(geon_server* s) {
                 s->running.push_back(r);
             }
         }
-        if (s->running.empty()) continue;
-        // ---- one quantum per active queue, kinds on their own streams ----
-        const double t0 = now_ms();
-        hipEventRecord(s->fork_ev, s->stream);
+        // ---- start idle pipelines that have work ----
         for (int k = 0; k < 4; ++k) {
-            Queue& Q64 = s->q[k][0];
-            Queue& Q32 = s->q[k][1];
-            if (!Q64.outstanding && !Q32.outstanding) continue;
-            hipStream_t st = s->kind_stream[k];
-            hipStreamWaitEvent(st, s->fork_ev, 0);
-            unsigned long long* ck = s->d_ctr + (size_t)k * K_WORDS;
-            if (Q32.outstanding) {
-                launch_quantum(s, Q32, k, 1, ck + K_CUR32, Q32.end, nullptr, nullptr, st, Q32.d_jobs,
-                               Q32.d_base, Q32.n_jobs);
-                launch_quantum(s, s->fb[k], k, 0, ck + K_FBCUR, 0, ck + K_FBCOUNT, s->d_list[k], st,
-                               Q32.d_jobs, Q32.d_base, Q32.n_jobs);
-            }
-            if (Q64.outstanding)
-                launch_quantum(s, Q64, k, 0, ck + K_CUR64, Q64.end, nullptr, nullptr, st, Q64.d_jobs,
-                               Q64.d_base, Q64.n_jobs);
-            hipEventRecord(s->join_ev[k], st);
-            hipStreamWaitEvent(s->stream, s->join_ev[k], 0);
-        }
-        s->quanta++;
-        // ---- completion counters of the running jobs ----
-        int lo = 1 << 30, hi = -1;
-        for (auto& r : s->running) {
-            lo = r->job < lo ? r->job : lo;
-            hi = r->job > hi ? r->job : hi;
-        }
-        const size_t nw = (size_t)(hi - lo + 1) * kJobWords;
-        if (s->h_job.size() < nw) s->h_job.resize(nw);
-        hipMemcpyDtoHAsync(s->h_job.data(), (hipDeviceptr_t)(s->d_job + (size_t)lo * kJobWords), nw * 8,
-                           s->stream);
-        hipStreamSynchronize(s->stream);
-        s->busy_ms += now_ms() - t0;
-        // ---- harvest ----
-        std::vector<std::shared_ptr<Request>> still;
-        for (auto& r : s->running) {
-            const unsigned long long done = s->h_job[(size_t)(r->job - lo) * kJobWords];
-            if (done < r->n_rays) {
-                still.push_back(r);
+            Pipe& P = s->pipe[k];
+            if (P.busy || (!s->q[k][0].outstanding && !s->q[k][1].outstanding &&
+                           !s->q[k][2].outstanding))
                 continue;
+            if (launch_kind(s, k) != GEON_OK) {
+                std::lock_guard<std::mutex> lk(s->mu);
+                s->err = GEON_E_HIP;
             }
-            s->q[r->mk][r->mq].outstanding--;
-            int st = GEON_OK;
-            bool more = false;
-            if (r->kind == GEON_REQ_RENDER) {
-                st = finish_render(s, *r);
-            } else {
-                st = finish_round(s, *r, &more);
-            }
-            if (more && st == GEON_OK) {
-                still.push_back(r);
-                continue;
-            }
-            std::lock_guard<std::mutex> lk(s->mu);
-            r->result.status = st;
-            r->result.ray_steps = r->ray_steps;
-            r->t_done = now_ms();
-            r->result.latency_ms = r->t_done - r->t_submit;
-            r->result.queue_ms = r->t_admit - r->t_submit;
-            r->state = 2;
-            if (!r->keep || r->kind == GEON_REQ_BISECT) free_req(s, *r);
-            s->n_done++;
-            s->steps_done += r->ray_steps;
-            s->done_cv.notify_all();
         }
-        s->running.swap(still);
+        any_busy = false;
+        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy;
+        if (any_busy && busy_since < 0.0) busy_since = now_ms();
+        // ---- harvest pipelines whose quantum ended ----
+        bool progressed = false;
+        for (int k = 0; k < 4; ++k) {
+            Pipe& P = s->pipe[k];
+            if (!P.busy || hipEventQuery(P.done) != hipSuccess) continue;
+            P.busy = false;
+            progressed = true;
+            if (P.ran32) P.parked32 = P.h_ctr[K_PARK32];
+            if (P.ranfb) P.parkedfb = P.h_ctr[K_PARKFB];
+            harvest_kind(s, k);
+        }
+        any_busy = false;
+        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy;
+        if (!any_busy && busy_since >= 0.0) {
+            s->busy_ms += now_ms() - busy_since;
+            busy_since = -1.0;
+        }
+        if (any_busy && !progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
 
@@ -607,16 +685,25 @@ int geon_serve_create(geon_ctx_t* ctx, const geon_serve_cfg_t* cfg, geon_server_
     if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(GEON_E_HIP, "stream"));
     for (int k = 0; k < 4; ++k) {   // default-priority streams, one per metric kind
-        hipStreamCreateWithFlags(&s->kind_stream[k], hipStreamNonBlocking);
-        hipEventCreateWithFlags(&s->join_ev[k], hipEventDisableTiming);
+        Pipe& P = s->pipe[k];
+        if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&P.done, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void**)&P.h_ctr, K_WORDS * 8, hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&P.h_job, (size_t)s->cfg.max_jobs * kJobWords * 8,
+                          hipHostMallocDefault) != hipSuccess)
+            return bail(fail(GEON_E_HIP, "pipeline resources"));
+        memset(P.h_ctr, 0, K_WORDS * 8);
     }
-    hipEventCreateWithFlags(&s->fork_ev, hipEventDisableTiming);
     if (hipMalloc(&s->arena_mem, s->cfg.hbm_budget) != hipSuccess)
         return bail(fail(GEON_E_NOMEM, "arena of %llu bytes", (unsigned long long)s->cfg.hbm_budget));
     s->arena.init((char*)s->arena_mem, s->cfg.hbm_budget);
     for (int k = 0; k < 4 && st == GEON_OK; ++k) {
-        for (int q = 0; q < 2 && st == GEON_OK; ++q) st = init_queue(s, s->q[k][q], k, q, s->cfg.max_jobs);
-        if (st == GEON_OK) st = init_queue(s, s->fb[k], k, 0, 1);
+        for (int q = 0; q < 2 && st == GEON_OK; ++q)
+            st = init_queue(s, s->q[k][q], k, q, s->cfg.max_jobs, false);
+        if (st == GEON_OK && k == GEON_METRIC_SCHWARZSCHILD)
+            st = init_queue(s, s->q[k][2], k, 0, s->cfg.max_jobs, true);
+        // hand-back quanta often run at low occupancy: one ray per lane
+        if (st == GEON_OK) st = init_queue(s, s->fb[k], k, 0, 1, true);
     }
     if (st != GEON_OK) return bail(st);
     // FP64 hand-back rings: unclaimed entries never exceed the rays admitted,
@@ -651,7 +738,7 @@ void geon_serve_destroy(geon_server_t* s) {
     if (s->stream) hipStreamSynchronize(s->stream);
     hipDeviceSynchronize();
     for (int k = 0; k < 4; ++k) {
-        for (int q = 0; q < 2; ++q) free_queue(s->q[k][q]);
+        for (int q = 0; q < 3; ++q) free_queue(s->q[k][q]);
         free_queue(s->fb[k]);
         if (s->d_list[k]) hipFree(s->d_list[k]);
     }
@@ -659,10 +746,12 @@ void geon_serve_destroy(geon_server_t* s) {
     if (s->d_job) hipFree(s->d_job);
     if (s->arena_mem) hipFree(s->arena_mem);
     for (int k = 0; k < 4; ++k) {
-        if (s->kind_stream[k]) hipStreamDestroy(s->kind_stream[k]);
-        if (s->join_ev[k]) hipEventDestroy(s->join_ev[k]);
+        Pipe& P = s->pipe[k];
+        if (P.stream) hipStreamDestroy(P.stream);
+        if (P.done) hipEventDestroy(P.done);
+        if (P.h_ctr) hipHostFree(P.h_ctr);
+        if (P.h_job) hipHostFree(P.h_job);
     }
-    if (s->fork_ev) hipEventDestroy(s->fork_ev);
     if (s->stream) hipStreamDestroy(s->stream);
     delete s;
 }

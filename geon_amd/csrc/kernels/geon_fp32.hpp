@@ -11,9 +11,10 @@
 // are handed to the FP64 path: a priori those whose impact parameter lies
 // within a band of a photon orbit; a posteriori those that end diverged or
 // flung far past the escape sphere, Schwarzschild captures (the reference
-// decides them at a near-singular stage), and those whose trajectory passed
-// within |sin θ| < 0.02 of the coordinate pole or dipped inside 3.1M (0.1 b0
-// at a wormhole throat), where FP32 errors are amplified.
+// decides them at a near-singular stage), those that dipped inside 3.2M
+// (0.1 b0 at a wormhole throat), and those that passed near the coordinate
+// pole: |sin θ| < 0.02 anywhere, or |sin θ| < 0.1 within 6M of a black hole
+// (everywhere for the wormhole), where FP32 errors are amplified.
 #pragma once
 #include "geon_device.hpp"
 
@@ -25,11 +26,18 @@ struct FParams {
     float R, inv_s, v, ship_x, inv_denom_s, denom_inv;
     float b0sq;
     float near_orbit;   // a ray dipping below this radius (|l|) is re-traced in FP64
+    float pole_r;       // within this radius (|l|) the wide pole band applies
 };
 
-// a ray whose |sin θ| drops below this passes the coordinate pole, where the
-// spherical chart is ill-conditioned in FP32
+// A ray whose |sin θ| drops below GEON_POLE_SIN passes the coordinate pole,
+// where the spherical chart is ill-conditioned: the reference's fixed step
+// turns sharply there and FP32 rounding changes the turn. Close to the hole
+// the angular rates are larger and the band widens to GEON_POLE_SIN_NEAR
+// (Schwarzschild 4096², r = 30M: escape directions off by radians at
+// |sin θ| ≈ 0.02, by 8e-3 rad at 0.05 and 4M, below 5e-4 rad beyond 0.1
+// or 5M).
 #define GEON_POLE_SIN 0.02f
+#define GEON_POLE_SIN_NEAR 0.1f
 
 GEON_D FParams fparams(const geon_dmetric& P) {
     FParams f;
@@ -45,8 +53,14 @@ GEON_D FParams fparams(const geon_dmetric& P) {
     f.denom_inv = (float)(1.0 / P.denom);              // 1 / (2 tanh(R/s))
     f.inv_denom_s = (float)(P.inv_s / P.denom);        // 1 / (2 s tanh(R/s))
     f.b0sq = (float)(P.b0 * P.b0);
-    f.near_orbit = P.kind == 1 ? (float)(3.1 * P.M) : (P.kind == 3 ? (float)(0.1 * P.b0) : 0.0f);
+    f.near_orbit = P.kind == 1 ? (float)(3.2 * P.M) : (P.kind == 3 ? (float)(0.1 * P.b0) : 0.0f);
+    f.pole_r = P.kind == 1 ? (float)(6.0 * P.M) : 1e30f;
     return f;
+}
+
+// the path came near the pole (min |sin θ|) with the closest radius min_r
+GEON_D bool pole_pass(const FParams& P, float min_s, float min_r) {
+    return min_s < GEON_POLE_SIN || (min_s < GEON_POLE_SIN_NEAR && min_r < P.pole_r);
 }
 
 GEON_D float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
@@ -122,7 +136,7 @@ struct SchwarzschildF {
     GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float min_s, float min_r) {
         if (cls == DIVERGED || cls == CAPTURED) return true;
         if (cls == ESCAPED && y[1] > 2.0f * P.esc_r) return true;
-        return min_s < GEON_POLE_SIN || min_r < P.near_orbit;
+        return pole_pass(P, min_s, min_r) || min_r < P.near_orbit;
     }
     GEON_D static float radius(const float* y) { return y[1]; }
 };
@@ -156,7 +170,7 @@ struct MorrisThorneF {
     GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float min_s, float min_r) {
         if (cls == DIVERGED) return true;
         if (cls == ESCAPED && fabsf(y[1]) > 2.0f * P.esc_l) return true;
-        return min_s < GEON_POLE_SIN || min_r < P.near_orbit;
+        return pole_pass(P, min_s, min_r) || min_r < P.near_orbit;
     }
     GEON_D static float radius(const float* y) { return fabsf(y[1]); }
 };

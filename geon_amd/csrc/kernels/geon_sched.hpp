@@ -330,11 +330,13 @@ __device__ void sched_run(const geon_sched_args& A) {
     }
 
     // park rays still in flight
+    uint32_t in_flight = 0;
     if (A.slot_g) {
 #pragma unroll
         for (int r = 0; r < RPL; ++r) {
             const uint32_t s = lane * RPL + r;
             if (act[r]) {
+                ++in_flight;
                 A.slot_g[s] = g[r];
                 A.slot_n[s] = n[r];
                 T* sy = (T*)A.slot_y;
@@ -349,9 +351,13 @@ __device__ void sched_run(const geon_sched_args& A) {
             }
         }
     }
-    // one atomic per wave for the executed-step counter
-    for (int off = 32; off > 0; off >>= 1) executed += __shfl_xor(executed, off);
+    // one atomic per wave for the executed-step and in-flight counters
+    for (int off = 32; off > 0; off >>= 1) {
+        executed += __shfl_xor(executed, off);
+        in_flight += __shfl_xor(in_flight, off);
+    }
     if (wl == 0 && A.ray_steps) atomicAdd(A.ray_steps, executed);
+    if (wl == 0 && A.parked && in_flight) atomicAdd(A.parked, (unsigned long long)in_flight);
 }
 
 }  // namespace geon

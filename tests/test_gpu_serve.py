@@ -3,8 +3,9 @@
 - Every request's result is the same alone and in a mix of concurrently
   admitted requests of other metrics, sizes and precisions (bit for bit:
   per-ray arithmetic never depends on slots or neighbours).
-- Shadow-bisection requests run as k-section rounds with the photon-sphere
-  early exit and reproduce b_c = 3√3 M (validate_shadow.rs:45-94).
+- Shadow-bisection requests run as k-section rounds: with photon-sphere
+  decisions they converge on b_c = 3√3 M; with full traces they reproduce
+  validate_shadow's own fixed-step result (validate_shadow.rs:45-94).
 - HBM admission: with a small budget requests wait for memory and still all
   complete; a request larger than the budget is refused with an error.
 """
