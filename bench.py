@@ -38,10 +38,13 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 
-# Algorithmic FP64 operations per RK4 ray-step of the Morris–Thorne kernel:
-# the reference's generic FD Christoffel path with structural zeros removed
-# (add/sub/mul/div, sqrt and each sin count 1).
-MT_OPS_PER_STEP = 418
+
+def op_counts():
+    """FP64 operations per RK4 ray-step and per curvature point of the kernels'
+    own formulation, counted by geon_amd/opcount (add/sub/mul/div/sqrt/sin/cos/
+    tanh count 1; structural zeros are not executed and not counted)."""
+    with open(os.path.join(ROOT, "geon_amd", "opcount", "op_counts.json")) as f:
+        return json.load(f)
 
 
 def parse():
@@ -51,6 +54,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="one 1 GiB device-to-device copy (HBM counter calibration) and exit")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
     return ap.parse_args()
 
@@ -61,6 +66,15 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1"}))
         sys.exit(2)
     import torch
+    if args.copy_probe:
+        n = 1 << 28                                   # 2^28 floats = 1 GiB each way
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n}))
+        return
 
     import geon_amd as geon
 
@@ -117,11 +131,12 @@ def main():
         assert final[k]["ray_steps"] == stats[k]["ray_steps"], k
     mt_ms = final["morris_thorne_l15"]["kernel_ms"]
     mt_steps = int(final["morris_thorne_l15"]["ray_steps"])
-    achieved = mt_steps * MT_OPS_PER_STEP / (mt_ms * 1e-3) / 1e12
+    mt_ops = op_counts()["ray_step"]["morris_thorne"]["ops"]
+    achieved = mt_steps * mt_ops / (mt_ms * 1e-3) / 1e12
     roofline = {"bound": "valu_fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
-                "kernel": "geon_render_rays_mt", "ops_per_ray_step": MT_OPS_PER_STEP,
-                "kernel_ms": round(mt_ms, 3)}
+                "kernel": "geon_render_rays_mt", "ops_per_ray_step": mt_ops,
+                "ray_steps_per_launch": mt_steps, "kernel_ms": round(mt_ms, 3)}
     result = {
         "metric": "RK4 ray-steps/s on 1 GPU",
         "value": value,

@@ -5,6 +5,10 @@
 Outputs (git-ignored, shipped beside the sources):
   geon_amd/lib/geon_kernels.hsaco   hipcc --genco --offload-arch=gfx950
   geon_amd/lib/libgeon_hip.so       host runtime + C ABI (g++, links libamdhip64)
+  geon_amd/lib/opcount              FP64 operation counter of the device code
+
+`python -m geon_amd.build --opcount` also rewrites geon_amd/opcount/op_counts.json,
+the per-ray-step and per-point operation counts used for the roofline.
 """
 from __future__ import annotations
 
@@ -61,6 +65,29 @@ def build_host(out=None):
     return out
 
 
+OPCOUNT_SRC = os.path.join(HERE, "opcount", "opcount.cpp")
+OPCOUNT_JSON = os.path.join(HERE, "opcount", "op_counts.json")
+
+
+def build_opcount(out=None):
+    """Host build of the device header with a counting scalar in place of double."""
+    out = out or os.path.join(LIB, "opcount")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
+          OPCOUNT_SRC, "-o", out + ".tmp"])
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def run_opcount(write=False):
+    text = subprocess.run([build_opcount()], check=True, capture_output=True, text=True).stdout
+    if write:
+        with open(OPCOUNT_JSON, "w") as f:
+            f.write(text)
+    return text
+
+
 def build_all():
     build_kernels()
     build_host()
@@ -68,4 +95,6 @@ def build_all():
 
 if __name__ == "__main__":
     build_all()
+    if "--opcount" in sys.argv:
+        run_opcount(write=True)
     sys.exit(0)

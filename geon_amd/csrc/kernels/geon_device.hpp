@@ -104,11 +104,39 @@ struct FD {
         });
     }
 
+    // Γ^μ_αβ = 0.5 Σ_σ g^μσ S(σ,α,β) (metric/mod.rs:55-66); Γ^μ_βα equals
+    // Γ^μ_αβ bit for bit (S is a commutative sum of the same two terms
+    // minus a symmetric entry) and is copied.
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* gam) {
         double g0[16], gi[16], dgv[64];
         MP::g(P, x, g0);
         MP::ginv(P, g0, gi);
         dg(P, x, dgv);
+        sfor<4>([&](auto Mu) {
+            sfor<4>([&](auto A) {
+                sfor<4>([&](auto B) {
+                    constexpr int mu = CI(Mu), a = CI(A), b = CI(B);
+                    if constexpr (gam_nz(mu, a, b)) {
+                        if constexpr (b < a && gam_nz(mu, b, a)) {
+                            gam[mu * 16 + a * 4 + b] = gam[mu * 16 + b * 4 + a];
+                        } else {
+                            double s = 0.0;
+                            sfor<4>([&](auto Sg) {
+                                constexpr int sg = CI(Sg);
+                                if constexpr (MP::ginv_nz(mu, sg) && S_nz(sg, a, b))
+                                    s += gi[mu * 4 + sg] * S<sg, a, b>(dgv);
+                            });
+                            gam[mu * 16 + a * 4 + b] = 0.5 * s;
+                        }
+                    }
+                });
+            });
+        });
+    }
+
+    // The same assembly from a g⁻¹ and ∂g the caller computed itself (a
+    // metric whose ∂g is written out with shared subexpressions).
+    GEON_D static void assemble(const double* gi, const double* dgv, double* gam) {
         sfor<4>([&](auto Mu) {
             sfor<4>([&](auto A) {
                 sfor<4>([&](auto B) {
@@ -201,18 +229,23 @@ struct Schwarzschild {
         g[15] = r * r * st * st;
     }
     // schwarzschild.rs:61-107
+    // Shared subexpressions are computed once: 2M, r², r²f and r − 2M enter
+    // several entries with the same operands, and −M/(r²f) = −(M/(r²f)).
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
         const double r = x[1], m = P.M;
-        const double f = 1.0 - 2.0 * m / r;
+        const double two_m = 2.0 * m;
+        const double f = 1.0 - two_m / r;
         double sin_t, cos_t;
         sincos(x[2], &sin_t, &cos_t);
-        const double gtr = m / (r * r * f);
+        const double r2 = r * r;
+        const double gtr = m / (r2 * f);
+        const double rs = r - two_m;
         c[0 * 16 + 0 * 4 + 1] = gtr;
         c[0 * 16 + 1 * 4 + 0] = gtr;
-        c[1 * 16 + 0 * 4 + 0] = m * f / (r * r);
-        c[1 * 16 + 1 * 4 + 1] = -m / (r * r * f);
-        c[1 * 16 + 2 * 4 + 2] = -(r - 2.0 * m);
-        c[1 * 16 + 3 * 4 + 3] = -(r - 2.0 * m) * sin_t * sin_t;
+        c[1 * 16 + 0 * 4 + 0] = m * f / r2;
+        c[1 * 16 + 1 * 4 + 1] = -gtr;
+        c[1 * 16 + 2 * 4 + 2] = -rs;
+        c[1 * 16 + 3 * 4 + 3] = -rs * sin_t * sin_t;
         const double inv_r = 1.0 / r;
         c[2 * 16 + 1 * 4 + 2] = inv_r;
         c[2 * 16 + 2 * 4 + 1] = inv_r;
@@ -275,8 +308,27 @@ struct MorrisThorne {
         gi[10] = m0 * m5 * m15 / det;
         gi[15] = m0 * m5 * m10 / det;
     }
+    // The generic Γ of metric/mod.rs:51-69 with ∂g by central differences
+    // (metric/mod.rs:25-43) written out for this metric: g_θθ = b0² + l² and
+    // g_φφ = g_θθ sin²θ, so the centre and the l ± h evaluations share
+    // sin θ and b0², and only θ ± h needs new sines. Every entry is the value
+    // the generic loop produces.
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
-        FD<MorrisThorne>::christoffel(P, x, c);
+        const double l = x[1], th = x[2], h = P.fd_h, two_h = P.fd_two_h;
+        const double b2 = P.b0 * P.b0;
+        const double st = sin(th);
+        const double r2 = b2 + l * l;
+        double g0[16], gi[16], dgv[64];
+        g0[0] = -1.0; g0[5] = 1.0; g0[10] = r2;
+        g0[15] = r2 * st * st;
+        ginv(P, g0, gi);
+        const double lp = l + h, lm = l - h;
+        const double r2p = b2 + lp * lp, r2m = b2 + lm * lm;
+        dgv[1 * 16 + 2 * 4 + 2] = (r2p - r2m) / two_h;
+        dgv[1 * 16 + 3 * 4 + 3] = (r2p * st * st - r2m * st * st) / two_h;
+        const double sp = sin(th + h), sm = sin(th - h);
+        dgv[2 * 16 + 3 * 4 + 3] = (r2 * sp * sp - r2 * sm * sm) / two_h;
+        FD<MorrisThorne>::assemble(gi, dgv, c);
     }
     GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
         double c[64];

@@ -1,0 +1,141 @@
+// opcount.cpp — counts the FP64 operations of the device formulation.
+//
+// The gfx950 kernels' per-ray code (geon_device.hpp) is compiled for the
+// host with every `double` replaced by a counting scalar, then one RK4 step
+// per metric is evaluated. The tallies are the algorithmic operation counts of exactly
+// the arithmetic the kernels execute (structural zeros already removed):
+// add, sub, mul, div, sqrt, sin, cos, tanh count 1 each (sincos counts 2).
+// Negation, |x|, comparisons, max/min are listed but not counted: on
+// gfx950 they are operand modifiers or compare/select instructions.
+//
+// Build and run (prints JSON): see geon_amd/build.py (build_opcount).
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <type_traits>
+#include <utility>
+
+#include <hip/hip_runtime.h>
+
+struct OpTally {
+    unsigned long long add, sub, mul, div, sqrt, sin, cos, tanh, acos, atan2, fmod, floor;
+    unsigned long long neg, cmp, fabs, fmax;
+    unsigned long long counted() const {
+        return add + sub + mul + div + sqrt + sin + cos + tanh + acos + atan2 + fmod + floor;
+    }
+};
+static OpTally T;
+
+struct cnt_d {
+    double v;
+    cnt_d() = default;
+    cnt_d(double x) : v(x) {}
+    explicit operator double() const { return v; }
+    explicit operator float() const { return (float)v; }
+    explicit operator long long() const { return (long long)v; }
+    cnt_d& operator+=(cnt_d o) { ++T.add; v += o.v; return *this; }
+    cnt_d& operator-=(cnt_d o) { ++T.sub; v -= o.v; return *this; }
+    cnt_d& operator*=(cnt_d o) { ++T.mul; v *= o.v; return *this; }
+    cnt_d& operator/=(cnt_d o) { ++T.div; v /= o.v; return *this; }
+};
+inline cnt_d operator+(cnt_d a, cnt_d b) { ++T.add; return cnt_d(a.v + b.v); }
+inline cnt_d operator-(cnt_d a, cnt_d b) { ++T.sub; return cnt_d(a.v - b.v); }
+inline cnt_d operator*(cnt_d a, cnt_d b) { ++T.mul; return cnt_d(a.v * b.v); }
+inline cnt_d operator/(cnt_d a, cnt_d b) { ++T.div; return cnt_d(a.v / b.v); }
+inline cnt_d operator-(cnt_d a) { ++T.neg; return cnt_d(-a.v); }
+inline bool operator<(cnt_d a, cnt_d b) { ++T.cmp; return a.v < b.v; }
+inline bool operator>(cnt_d a, cnt_d b) { ++T.cmp; return a.v > b.v; }
+inline bool operator<=(cnt_d a, cnt_d b) { ++T.cmp; return a.v <= b.v; }
+inline bool operator>=(cnt_d a, cnt_d b) { ++T.cmp; return a.v >= b.v; }
+inline bool operator==(cnt_d a, cnt_d b) { ++T.cmp; return a.v == b.v; }
+inline bool operator!=(cnt_d a, cnt_d b) { ++T.cmp; return a.v != b.v; }
+inline cnt_d sqrt(cnt_d a) { ++T.sqrt; return cnt_d(::sqrt(a.v)); }
+inline cnt_d sin(cnt_d a) { ++T.sin; return cnt_d(::sin(a.v)); }
+inline cnt_d cos(cnt_d a) { ++T.cos; return cnt_d(::cos(a.v)); }
+inline void sincos(cnt_d a, cnt_d* s, cnt_d* c) {
+    ++T.sin;
+    ++T.cos;
+    *s = cnt_d(::sin(a.v));
+    *c = cnt_d(::cos(a.v));
+}
+inline cnt_d tanh(cnt_d a) { ++T.tanh; return cnt_d(::tanh(a.v)); }
+inline cnt_d acos(cnt_d a) { ++T.acos; return cnt_d(::acos(a.v)); }
+inline cnt_d atan2(cnt_d a, cnt_d b) { ++T.atan2; return cnt_d(::atan2(a.v, b.v)); }
+inline cnt_d fmod(cnt_d a, cnt_d b) { ++T.fmod; return cnt_d(::fmod(a.v, b.v)); }
+inline cnt_d floor(cnt_d a) { ++T.floor; return cnt_d(::floor(a.v)); }
+inline cnt_d fabs(cnt_d a) { ++T.fabs; return cnt_d(::fabs(a.v)); }
+inline cnt_d fmax(cnt_d a, cnt_d b) { ++T.fmax; return cnt_d(::fmax(a.v, b.v)); }
+inline bool cnt_isfinite(cnt_d a) { return ::isfinite(a.v); }
+inline bool cnt_isnan(cnt_d a) { return ::isnan(a.v); }
+
+#define __builtin_isfinite(x) cnt_isfinite(x)
+#define __builtin_isnan(x) cnt_isnan(x)
+#define double cnt_d
+#include "../csrc/kernels/geon_device.hpp"
+#undef double
+#undef __builtin_isfinite
+#undef __builtin_isnan
+
+using namespace geon;
+
+static geon_dmetric metric(int kind) {
+    geon_dmetric P;
+    memset(&P, 0, sizeof P);
+    P.kind = kind;
+    P.M = 1.0;
+    P.cap_r = 1.005 * 2.0;
+    P.esc_r = 100.0;
+    P.esc2 = 1e6;
+    P.esc_l = 1000.0;
+    P.fd_h = 1e-5;
+    P.fd_two_h = 2e-5;
+    P.fd_inv_2h = 1.0 / 2e-5;
+    P.b0 = 1.0;
+    P.R = 2.0;
+    P.s = 0.125;
+    P.s_pow2 = 1;
+    P.inv_s = 8.0;
+    P.v = 0.5;
+    P.ship_x = 0.0;
+    P.denom = 2.0 * ::tanh(16.0);
+    P.flat_rs = 2.0 + 22.0 * 0.125 + 2e-5;
+    return P;
+}
+
+template <class MP>
+static OpTally step_ops(const geon_dmetric& P, const double* y0) {
+    cnt_d y[8], out[8];
+    for (int i = 0; i < 8; ++i) y[i] = y0[i];
+    memset(&T, 0, sizeof T);
+    rk4_step<MP>(P, y, cnt_d(0.05), out);
+    return T;
+}
+
+static void emit(const char* name, const OpTally& t, bool last) {
+    printf("    \"%s\": {\"ops\": %llu, \"add\": %llu, \"sub\": %llu, \"mul\": %llu, \"div\": %llu, "
+           "\"sqrt\": %llu, \"sin\": %llu, \"cos\": %llu, \"tanh\": %llu, "
+           "\"not_counted\": {\"neg\": %llu, \"cmp\": %llu, \"fabs\": %llu, \"fmax\": %llu}}%s\n",
+           name, t.counted(), t.add, t.sub, t.mul, t.div, t.sqrt, t.sin, t.cos, t.tanh, t.neg, t.cmp,
+           t.fabs, t.fmax, last ? "" : ",");
+}
+
+int main() {
+    // representative states: off the pole (cot branch taken), inside the
+    // Alcubierre wall (full Γ) and outside it (exact flat early-out)
+    const double ys[8] = {0.0, 10.0, 1.2, 0.3, 1.3, -0.4, 0.02, 0.03};
+    const double ym[8] = {0.0, 5.0, 1.2, 0.3, 1.0, -0.4, 0.02, 0.03};
+    const double ya_wall[8] = {0.0, 2.0, 0.3, 0.2, 1.2, -0.5, 0.3, 0.1};
+    const double ya_flat[8] = {0.0, 8.0, 0.3, 0.2, 1.2, -0.5, 0.3, 0.1};
+    const double yk[8] = {0.0, 5.0, 1.0, 2.0, 1.0, -0.6, 0.0, 0.8};
+    printf("{\n  \"unit\": \"FP64 operations per RK4 ray-step (rk4_step of geon_device.hpp)\",\n");
+    printf("  \"ray_step\": {\n");
+    emit("minkowski", step_ops<Minkowski>(metric(0), yk), false);
+    emit("schwarzschild", step_ops<Schwarzschild>(metric(1), ys), false);
+    emit("morris_thorne", step_ops<MorrisThorne>(metric(3), ym), false);
+    emit("alcubierre_wall", step_ops<Alcubierre>(metric(2), ya_wall), false);
+    emit("alcubierre_flat", step_ops<Alcubierre>(metric(2), ya_flat), true);
+    printf("  }\n}\n");
+    return 0;
+}
