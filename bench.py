@@ -39,6 +39,21 @@ sys.path.insert(0, ROOT)
 FP64_PEAK_TFLOPS = 78.6
 
 
+def measured_traffic():
+    """HBM bytes per launch of geon_render_rays_mt on this workload, measured by
+    rocprofv3 FETCH_SIZE/WRITE_SIZE passes scaled by a 1 GiB copy (the summary
+    and its commands are in profiles/render_mt_fp64_hbm_traffic.txt)."""
+    path = os.path.join(ROOT, "profiles", "render_mt_fp64_hbm_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def op_counts():
     """FP64 operations per RK4 ray-step and per curvature point of the kernels'
     own formulation, counted by geon_amd/opcount (add/sub/mul/div/sqrt/sin/cos/
@@ -134,7 +149,8 @@ def main():
     mt_ops = op_counts()["ray_step"]["morris_thorne"]["ops"]
     achieved = mt_steps * mt_ops / (mt_ms * 1e-3) / 1e12
     roofline = {"bound": "valu_fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
+                "traffic": measured_traffic(), "algorithmic_bytes": 40 * 1024 * 1024,
                 "kernel": "geon_render_rays_mt", "ops_per_ray_step": mt_ops,
                 "ray_steps_per_launch": mt_steps, "kernel_ms": round(mt_ms, 3)}
     result = {

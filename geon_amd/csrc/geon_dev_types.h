@@ -23,6 +23,7 @@ struct geon_dmetric {
     double  fd_h;        // metric/mod.rs:73-75
     double  fd_two_h;    // 2.0 * h                           (metric/mod.rs:35)
     double  fd_inv_2h;   // 1.0 / (2.0 * h)                   (curvature.rs:34)
+    double  fd_sin_h, fd_cos_h;   // sin h, cos h: θ ± h rotations (Morris–Thorne)
 };
 
 struct geon_dframe {

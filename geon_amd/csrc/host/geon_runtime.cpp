@@ -146,7 +146,11 @@ int make_dmetric(const geon_metric_t* m, geon_dmetric* d) {
         // sees f == 0.
         d->flat_rs = d->R + 22.0 * fabs(d->s) + 2.0 * m->fd_step;
     }
-    if (m->kind == GEON_METRIC_MORRIS_THORNE) d->b0 = m->p[0];
+    if (m->kind == GEON_METRIC_MORRIS_THORNE) {
+        d->b0 = m->p[0];
+        d->fd_sin_h = sin(m->fd_step);
+        d->fd_cos_h = cos(m->fd_step);
+    }
     return GEON_OK;
 }
 

@@ -8,7 +8,9 @@
 // skip masked terms at compile time. Adding an exact zero never changes an
 // IEEE sum, so the skipped work leaves the results identical to the dense
 // reference loops (geodesic.rs:49-57, metric/mod.rs:55-66) while costing
-// only the nonzero terms. The FP64 build uses no FMA contraction.
+// only the nonzero terms. The FP64 build uses no FMA contraction. One
+// deliberate exception: Morris–Thorne's sin(θ ± h) come from angle addition
+// (see MorrisThorne::christoffel).
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -311,12 +313,17 @@ struct MorrisThorne {
     // The generic Γ of metric/mod.rs:51-69 with ∂g by central differences
     // (metric/mod.rs:25-43) written out for this metric: g_θθ = b0² + l² and
     // g_φφ = g_θθ sin²θ, so the centre and the l ± h evaluations share
-    // sin θ and b0², and only θ ± h needs new sines. Every entry is the value
-    // the generic loop produces.
+    // sin θ and b0². sin(θ ± h) comes from one sincos θ rotated by the
+    // constants sin h, cos h (angle addition) instead of two more sines:
+    // each differs from a directly evaluated sine by at most an ulp, the
+    // same size as one libm against another, so ∂_θ g_φφ moves by
+    // ~1e-11 relative (its FD quotient amplifies by 1/h). Every other entry
+    // is the value the generic loop produces.
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
         const double l = x[1], th = x[2], h = P.fd_h, two_h = P.fd_two_h;
         const double b2 = P.b0 * P.b0;
-        const double st = sin(th);
+        double st, ct;
+        sincos(th, &st, &ct);
         const double r2 = b2 + l * l;
         double g0[16], gi[16], dgv[64];
         g0[0] = -1.0; g0[5] = 1.0; g0[10] = r2;
@@ -326,7 +333,8 @@ struct MorrisThorne {
         const double r2p = b2 + lp * lp, r2m = b2 + lm * lm;
         dgv[1 * 16 + 2 * 4 + 2] = (r2p - r2m) / two_h;
         dgv[1 * 16 + 3 * 4 + 3] = (r2p * st * st - r2m * st * st) / two_h;
-        const double sp = sin(th + h), sm = sin(th - h);
+        const double sp = st * P.fd_cos_h + ct * P.fd_sin_h;   // sin(θ + h)
+        const double sm = st * P.fd_cos_h - ct * P.fd_sin_h;   // sin(θ − h)
         dgv[2 * 16 + 3 * 4 + 3] = (r2 * sp * sp - r2 * sm * sm) / two_h;
         FD<MorrisThorne>::assemble(gi, dgv, c);
     }

@@ -92,6 +92,8 @@ static geon_dmetric metric(int kind) {
     P.fd_h = 1e-5;
     P.fd_two_h = 2e-5;
     P.fd_inv_2h = 1.0 / 2e-5;
+    P.fd_sin_h = ::sin(1e-5);
+    P.fd_cos_h = ::cos(1e-5);
     P.b0 = 1.0;
     P.R = 2.0;
     P.s = 0.125;

@@ -35,9 +35,10 @@ def test_counts_scale_with_metric_work(counts):
     # analytic Γ < generic FD Γ with one sine per shift < FD Γ with tanh walls
     assert rs["minkowski"]["ops"] < rs["schwarzschild"]["ops"] < rs["morris_thorne"]["ops"] \
         < rs["alcubierre_wall"]["ops"]
-    # Schwarzschild: one sincos per stage; Morris-Thorne: sin θ, sin(θ ± h) per stage
+    # one sincos per stage: Schwarzschild for Γ, Morris-Thorne for sin θ and,
+    # by angle addition, sin(θ ± h)
     assert rs["schwarzschild"]["sin"] == 4 and rs["schwarzschild"]["cos"] == 4
-    assert rs["morris_thorne"]["sin"] == 12
+    assert rs["morris_thorne"]["sin"] == 4 and rs["morris_thorne"]["cos"] == 4
     # beyond the Alcubierre wall Γ is exactly zero: the step is the combination
     # plus the r_s test (3 mul, 2 add, 1 sqrt, 1 sub per stage)
     assert rs["alcubierre_flat"]["ops"] == 106 + 4 * 7
