@@ -69,6 +69,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
+    ap.add_argument("--sections", default="split,ragged,serving,curvature",
+                    help="comma-separated extra sections to run after the headline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 1 GiB device-to-device copy (HBM counter calibration) and exit")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
@@ -176,13 +178,17 @@ def main():
         "wall_s": round(wall, 3),
     }
     log(f"headline: {value:.4e} ray-steps/s, {ms_per_step:.1f} ms per step")
-    if not args.quick:
+    sections = set() if args.quick else set(args.sections.split(","))
+    if "split" in sections:
         result["precision_split"] = precision_split(geon, ctx, torch, stream, outs, ev)
         log(f"precision split: {json.dumps(result['precision_split'])}")
+    if "ragged" in sections:
         result["ragged_batch"] = ragged_batch(geon, ctx, torch, stream, ev)
         log(f"ragged batch: {json.dumps(result['ragged_batch'])}")
+    if "serving" in sections:
         result["serving"] = serving(geon, ctx, torch, stream, outs, ev, args.serving_seconds)
         log(f"serving: {json.dumps(result['serving'])}")
+    if "curvature" in sections:
         result["curvature"] = curvature(geon, ctx, torch, stream, ev, alc, mt)
         log(f"curvature: {json.dumps(result['curvature'])}")
     if not args.no_cpu_baseline:

@@ -35,6 +35,27 @@ GEON_D void sfor(F&& f) {
 }
 #define CI(v) decltype(v)::value
 
+// RN(a / b) given y = RN(1 / b), without a division (Markstein's theorem):
+// q = a·y lies within an ulp of a/b, the remainder a − b·q is exact in one
+// FMA, and q + (a − b·q)·y rounds to RN(a/b). A zero remainder means q is
+// exact and already carries the right sign of zero. Valid while a, b and
+// the quotient are normal numbers far from overflow, which each caller
+// guarantees or guards. Saves the ~11-instruction IEEE division sequence
+// where one reciprocal serves several quotients or is a constant.
+#ifndef GEON_OPCOUNT_DIV_RN   // the op-counting build tallies these as one division
+GEON_D double div_rn(double a, double b, double y) {
+    const double q = a * y;
+    const double r = __builtin_fma(-q, b, a);
+    return r == 0.0 ? q : __builtin_fma(r, y, q);
+}
+// The same for a numerator that is never −0: then q + r·y is exact for an
+// exact q too (a +0 quotient stays +0), and the zero-remainder select goes.
+GEON_D double div_rn_pz(double a, double b, double y) {
+    const double q = a * y;
+    return __builtin_fma(__builtin_fma(-q, b, a), y, q);
+}
+#endif
+
 GEON_D bool finite8(const double* y) {
     bool ok = true;
 #pragma unroll
@@ -193,6 +214,7 @@ GEON_D double dot4(const double* g, const double* a, const double* b) {
 
 // ---- Minkowski (minkowski.rs) ---------------------------------------------
 struct Minkowski {
+    static constexpr int flat_test = 1;   // Γ ≡ 0: every step is a flat step
     static constexpr bool g_nz(int m, int n) { return m == n; }
     static constexpr bool ginv_nz(int m, int s) { return m == s; }
     static constexpr bool gam_nz(int, int, int) { return false; }
@@ -213,6 +235,7 @@ struct Minkowski {
 
 // ---- Schwarzschild (schwarzschild.rs) -------------------------------------
 struct Schwarzschild {
+    static constexpr int flat_test = 0;
     static constexpr bool g_nz(int m, int n) { return m == n; }
     static constexpr bool gam_nz(int mu, int a, int b) {
         return (mu == 0 && ((a == 0 && b == 1) || (a == 1 && b == 0))) ||
@@ -236,7 +259,10 @@ struct Schwarzschild {
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
         const double r = x[1], m = P.M;
         const double two_m = 2.0 * m;
-        const double f = 1.0 - two_m / r;
+        const double inv_r = 1.0 / r;
+        // 2M/r from 1/r; a non-finite r or 1/r leaves f non-finite as the
+        // IEEE division would, and the step is discarded as diverged either way
+        const double f = 1.0 - div_rn(two_m, r, inv_r);
         double sin_t, cos_t;
         sincos(x[2], &sin_t, &cos_t);
         const double r2 = r * r;
@@ -248,7 +274,6 @@ struct Schwarzschild {
         c[1 * 16 + 1 * 4 + 1] = -gtr;
         c[1 * 16 + 2 * 4 + 2] = -rs;
         c[1 * 16 + 3 * 4 + 3] = -rs * sin_t * sin_t;
-        const double inv_r = 1.0 / r;
         c[2 * 16 + 1 * 4 + 2] = inv_r;
         c[2 * 16 + 2 * 4 + 1] = inv_r;
         c[2 * 16 + 3 * 4 + 3] = -sin_t * cos_t;
@@ -286,6 +311,7 @@ struct Schwarzschild {
 
 // ---- Morris–Thorne (morris_thorne.rs), generic FD Γ -----------------------
 struct MorrisThorne {
+    static constexpr int flat_test = 0;
     static constexpr bool g_nz(int m, int n) { return m == n; }
     static constexpr bool dg_nz(int a, int m, int n) {
         return (a == 1 && m == n && (m == 2 || m == 3)) || (a == 2 && m == 3 && n == 3);
@@ -305,10 +331,22 @@ struct MorrisThorne {
         const double m0 = g[0], m5 = g[5], m10 = g[10], m15 = g[15];
         const double c00 = m5 * m10 * m15;
         const double det = m0 * c00;
-        gi[0] = c00 / det;
-        gi[5] = m0 * m10 * m15 / det;
-        gi[10] = m0 * m5 * m15 / det;
-        gi[15] = m0 * m5 * m10 / det;
+        const double n5 = m0 * m10 * m15, n10 = m0 * m5 * m15, n15 = m0 * m5 * m10;
+        const double ad = fabs(det);
+        if (__builtin_expect(ad > 0x1p-500 && ad < 0x1p500, 1)) {
+            // one division serves the four quotients; with det ≠ 0 every
+            // numerator is a nonzero product (g_θθ = b0² + l² > 0, g_φφ ≠ 0)
+            const double y = 1.0 / det;
+            gi[0] = div_rn_pz(c00, det, y);
+            gi[5] = div_rn_pz(n5, det, y);
+            gi[10] = div_rn_pz(n10, det, y);
+            gi[15] = div_rn_pz(n15, det, y);
+        } else {
+            gi[0] = c00 / det;
+            gi[5] = n5 / det;
+            gi[10] = n10 / det;
+            gi[15] = n15 / det;
+        }
     }
     // The generic Γ of metric/mod.rs:51-69 with ∂g by central differences
     // (metric/mod.rs:25-43) written out for this metric: g_θθ = b0² + l² and
@@ -331,11 +369,16 @@ struct MorrisThorne {
         ginv(P, g0, gi);
         const double lp = l + h, lm = l - h;
         const double r2p = b2 + lp * lp, r2m = b2 + lm * lm;
-        dgv[1 * 16 + 2 * 4 + 2] = (r2p - r2m) / two_h;
-        dgv[1 * 16 + 3 * 4 + 3] = (r2p * st * st - r2m * st * st) / two_h;
+        // quotients by 2h from the exact host reciprocal 1/(2h); metric values
+        // here are O(b0² + l²), far from overflow in every state that can
+        // still escape, and each numerator is a difference of two
+        // non-negative values, never −0
+        const double i2h = P.fd_inv_2h;
+        dgv[1 * 16 + 2 * 4 + 2] = div_rn_pz(r2p - r2m, two_h, i2h);
+        dgv[1 * 16 + 3 * 4 + 3] = div_rn_pz(r2p * st * st - r2m * st * st, two_h, i2h);
         const double sp = st * P.fd_cos_h + ct * P.fd_sin_h;   // sin(θ + h)
         const double sm = st * P.fd_cos_h - ct * P.fd_sin_h;   // sin(θ − h)
-        dgv[2 * 16 + 3 * 4 + 3] = (r2 * sp * sp - r2 * sm * sm) / two_h;
+        dgv[2 * 16 + 3 * 4 + 3] = div_rn_pz(r2 * sp * sp - r2 * sm * sm, two_h, i2h);
         FD<MorrisThorne>::assemble(gi, dgv, c);
     }
     GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
@@ -358,6 +401,7 @@ struct MorrisThorne {
 
 // ---- Alcubierre (alcubierre.rs), generic FD Γ -----------------------------
 struct Alcubierre {
+    static constexpr int flat_test = 2;   // flat steps decided per step by flat()
     static constexpr bool g_nz(int m, int n) {
         return m == n || (m == 0 && n == 1) || (m == 1 && n == 0);
     }
@@ -428,13 +472,54 @@ struct Alcubierre {
         if (!__builtin_isfinite(r2)) return DIVERGED;
         return (r2 > P.esc2) ? ESCAPED : RUNNING;
     }
+    // True when all four RK4 stages of the step from y see the exact flat
+    // region (accel's early-out). While Γ = 0 the stage momenta equal p, so
+    // every stage position lies within |h|·|p_spatial| of y and its r_s is at
+    // least r_s(y) − |h|·|p|; the 1e-9 margin covers the rounding of the
+    // stage positions and of r_s. NaN or infinite states fail the test.
+    GEON_D static bool flat(const geon_dmetric& P, const double* y, double h) {
+        const double r = rs(P, y);
+        const double d = r - P.flat_rs - 1e-9 * (1.0 + r);
+        if (!(d > 0.0)) return false;
+        const double p2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
+        return d * d > h * h * p2;
+    }
 };
 
 // ---- integrator (integrator.rs:16-40) -------------------------------------
+// The step below with dp = −0 at every stage (what accel returns where Γ is
+// exactly zero), the zero terms folded: the same operations on the same
+// values, so the same bits. Stage momenta are p + hh·(−0) and p + h·(−0);
+// momenta advance by (h/6)·(−0).
+GEON_D void rk4_step_flat(const double* y, double h, double* out) {
+    const double hh = 0.5 * h;
+    const double zh = hh * -0.0, zf = h * -0.0;
+    const double sixth = h / 6.0;
+    const double zs = sixth * -0.0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double p = y[4 + i];
+        const double k23 = p + zh;
+        const double t = 2.0 * k23;
+        const double acc = ((p + t) + t) + (p + zf);
+        out[i] = y[i] + sixth * acc;
+        out[4 + i] = p + zs;
+    }
+}
+
 // k_i = (p_i, dp_i); the combination ((k1 + 2k2) + 2k3) + k4 is accumulated
-// in that order.
+// in that order. Metrics with flat regions take the folded step there.
 template <class MP>
 GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out) {
+    if constexpr (MP::flat_test == 1) {
+        rk4_step_flat(y, h, out);
+        return;
+    } else if constexpr (MP::flat_test == 2) {
+        if (MP::flat(P, y, h)) {
+            rk4_step_flat(y, h, out);
+            return;
+        }
+    }
     double acc[8], yt[8], dp[4];
     const double hh = 0.5 * h;
     MP::accel(P, y, dp);

@@ -7,7 +7,9 @@
 // (its O(h²) truncation, ~1e-10 relative, is far below FP32 resolution).
 // After each step p^t is re-projected onto the null cone g(p,p) = 0
 // (future-directed root), which keeps FP32 drift out of the
-// energy-carrying component. Rays whose outcome is sensitive to rounding
+// energy-carrying component. Where Γ vanishes at every stage (Minkowski,
+// Alcubierre beyond its wall) the step is the straight line y + h·p.
+// Rays whose outcome is sensitive to rounding
 // are handed to the FP64 path: a priori those whose impact parameter lies
 // within a band of a photon orbit; a posteriori those that end diverged or
 // flung far past the escape sphere, Schwarzschild captures (the reference
@@ -25,6 +27,7 @@ struct FParams {
     float M, cap_r, esc_r, esc2, esc_l;
     float R, inv_s, v, ship_x, inv_denom_s, denom_inv;
     float b0sq;
+    float flat_rs;      // Alcubierre: beyond this r_s both FP32 tanh are exactly 1
     float near_orbit;   // a ray dipping below this radius (|l|) is re-traced in FP64
     float pole_r;       // within this radius (|l|) the wide pole band applies
 };
@@ -53,6 +56,8 @@ GEON_D FParams fparams(const geon_dmetric& P) {
     f.denom_inv = (float)(1.0 / P.denom);              // 1 / (2 tanh(R/s))
     f.inv_denom_s = (float)(P.inv_s / P.denom);        // 1 / (2 s tanh(R/s))
     f.b0sq = (float)(P.b0 * P.b0);
+    // 1 − 2/(e^{2u} + 1) rounds to 1.0f for u ≥ 9.1; 10 wall widths of margin
+    f.flat_rs = (float)(P.R + 10.0 * fabs(P.s));
     f.near_orbit = P.kind == 1 ? (float)(3.2 * P.M) : (P.kind == 3 ? (float)(0.1 * P.b0) : 0.0f);
     f.pole_r = P.kind == 1 ? (float)(6.0 * P.M) : 1e30f;
     return f;
@@ -75,6 +80,8 @@ GEON_D float ftanh(float u) {
 
 // ---- Minkowski --------------------------------------------------------------
 struct MinkowskiF {
+    static constexpr int flat_test = 1;
+    GEON_D static bool flat(const FParams&, const float*, float) { return true; }
     GEON_D static void accel(const FParams&, const float*, float* dp, float, float) {
         dp[0] = dp[1] = dp[2] = dp[3] = 0.0f;
     }
@@ -97,6 +104,7 @@ struct MinkowskiF {
 // ---- Schwarzschild (analytic Γ, schwarzschild.rs:61-107) --------------------
 struct SchwarzschildF {
     static constexpr bool uses_theta = true;
+    static constexpr int flat_test = 0;
     GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
 #pragma clang fp contract(fast)
         const float r = y[1];
@@ -144,6 +152,7 @@ struct SchwarzschildF {
 // ---- Morris–Thorne (analytic ∂g of morris_thorne.rs:22-36) ------------------
 struct MorrisThorneF {
     static constexpr bool uses_theta = true;
+    static constexpr int flat_test = 0;
     GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
 #pragma clang fp contract(fast)
         const float l = y[1];
@@ -183,6 +192,15 @@ struct MorrisThorneF {
 //   dp⁰ = ½(Q0 + vf Q1), dp¹ = ½(vf Q0 − (1 − v²f²) Q1), dp^k = −½ Qk.
 struct AlcubierreF {
     static constexpr bool uses_theta = false;
+    static constexpr int flat_test = 2;
+    // Every stage of the step from y stays beyond flat_rs, where Γ is zero.
+    GEON_D static bool flat(const FParams& P, const float* y, float h) {
+#pragma clang fp contract(fast)
+        const float dx = y[1] - P.ship_x, dy = y[2], dz = y[3];
+        const float d = fsqrt(dx * dx + dy * dy + dz * dz) - P.flat_rs;
+        const float p2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
+        return d > 0.0f && d * d > h * h * p2;
+    }
     GEON_D static void shape(const FParams& P, const float* y, float& f, float& fp, float* n) {
 #pragma clang fp contract(fast)
         const float dx = y[1] - P.ship_x, dy = y[2], dz = y[3];
@@ -263,6 +281,18 @@ GEON_D void rotate_sc(float s, float c, float theta_new, float d, float& so, flo
 template <class MF>
 GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c) {
 #pragma clang fp contract(fast)
+    if constexpr (MF::flat_test != 0) {
+        // Γ = 0 at every stage: the RK4 step is the straight line y + h·p
+        // (momenta, p^t included, unchanged)
+        if (MF::flat_test == 1 || MF::flat(P, y, h)) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                out[i] = y[i] + h * y[4 + i];
+                out[4 + i] = y[4 + i];
+            }
+            return 1.0f;
+        }
+    }
     float acc[8], yt[8], dp[4];
     float ss = s, cs = c;
     const float hh = 0.5f * h;

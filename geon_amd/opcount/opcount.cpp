@@ -4,7 +4,8 @@
 // host with every `double` replaced by a counting scalar, then one RK4 step
 // per metric is evaluated. The tallies are the algorithmic operation counts of exactly
 // the arithmetic the kernels execute (structural zeros already removed):
-// add, sub, mul, div, sqrt, sin, cos, tanh count 1 each (sincos counts 2).
+// add, sub, mul, div, sqrt, sin, cos, tanh count 1 each (sincos counts 2,
+// an FMA counts as a mul and an add).
 // Negation, |x|, comparisons, max/min are listed but not counted: on
 // gfx950 they are operand modifiers or compare/select instructions.
 //
@@ -67,15 +68,26 @@ inline cnt_d fmod(cnt_d a, cnt_d b) { ++T.fmod; return cnt_d(::fmod(a.v, b.v)); 
 inline cnt_d floor(cnt_d a) { ++T.floor; return cnt_d(::floor(a.v)); }
 inline cnt_d fabs(cnt_d a) { ++T.fabs; return cnt_d(::fabs(a.v)); }
 inline cnt_d fmax(cnt_d a, cnt_d b) { ++T.fmax; return cnt_d(::fmax(a.v, b.v)); }
+inline cnt_d cnt_fma(cnt_d a, cnt_d b, cnt_d c) {
+    ++T.mul;
+    ++T.add;
+    return cnt_d(::fma(a.v, b.v, c.v));
+}
 inline bool cnt_isfinite(cnt_d a) { return ::isfinite(a.v); }
 inline bool cnt_isnan(cnt_d a) { return ::isnan(a.v); }
 
+// a quotient from a precomputed reciprocal is one algorithmic division
+inline cnt_d div_rn(cnt_d a, cnt_d b, cnt_d) { ++T.div; return cnt_d(a.v / b.v); }
+inline cnt_d div_rn_pz(cnt_d a, cnt_d b, cnt_d) { ++T.div; return cnt_d(a.v / b.v); }
+#define GEON_OPCOUNT_DIV_RN 1
 #define __builtin_isfinite(x) cnt_isfinite(x)
+#define __builtin_fma(a, b, c) cnt_fma(a, b, c)
 #define __builtin_isnan(x) cnt_isnan(x)
 #define double cnt_d
 #include "../csrc/kernels/geon_device.hpp"
 #undef double
 #undef __builtin_isfinite
+#undef __builtin_fma
 #undef __builtin_isnan
 
 using namespace geon;

@@ -2,10 +2,13 @@
 
 The counter compiles the kernels' per-ray code for the host with a counting
 scalar in place of double. The committed op_counts.json (used for the
-roofline in bench.py) must be what the current sources produce, and the
-Minkowski step, whose Christoffel symbols vanish, must cost exactly the RK4
-combination: hh = 0.5h (1), stage 2 inputs 4·(2+2), stages 3 and 4 inputs
-2·4·(2+2+2+2), h/6 (1), k4 accumulation 4·2 and the update 8·2 = 106.
+roofline in bench.py) must be what the current sources produce. Closed
+forms pin the counter itself: a Minkowski step is the folded zero-Γ RK4
+step, 5 shared ops (h/2, h/6 and three products with −0) plus 9 per
+component (stage momentum, 2k, three sums, p + h·(−0), (h/6)·acc, the
+position and the momentum update) = 41; an Alcubierre step outside the wall
+adds its flatness test: r_s (7), the margin (4), |p|² (5) and the
+comparison products (3) = 19.
 """
 import json
 import os
@@ -26,8 +29,8 @@ def test_committed_counts_are_current(counts):
     assert committed == counts
 
 
-def test_minkowski_step_is_the_rk4_combination(counts):
-    assert counts["ray_step"]["minkowski"]["ops"] == 1 + 16 + 2 * 32 + 1 + 8 + 16
+def test_minkowski_step_is_the_folded_rk4_step(counts):
+    assert counts["ray_step"]["minkowski"]["ops"] == 5 + 4 * 9
 
 
 def test_counts_scale_with_metric_work(counts):
@@ -39,6 +42,5 @@ def test_counts_scale_with_metric_work(counts):
     # by angle addition, sin(θ ± h)
     assert rs["schwarzschild"]["sin"] == 4 and rs["schwarzschild"]["cos"] == 4
     assert rs["morris_thorne"]["sin"] == 4 and rs["morris_thorne"]["cos"] == 4
-    # beyond the Alcubierre wall Γ is exactly zero: the step is the combination
-    # plus the r_s test (3 mul, 2 add, 1 sqrt, 1 sub per stage)
-    assert rs["alcubierre_flat"]["ops"] == 106 + 4 * 7
+    # beyond the Alcubierre wall Γ is exactly zero: the folded step plus the test
+    assert rs["alcubierre_flat"]["ops"] == 5 + 4 * 9 + 19
