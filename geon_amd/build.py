@@ -33,7 +33,8 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
 
 KERNEL_SRC = [os.path.join(CSRC, "kernels", "geon_kernels.hip")]
 KERNEL_DEPS = KERNEL_SRC + [os.path.join(CSRC, "kernels", f)
-                            for f in ("geon_device.hpp", "geon_fp32.hpp", "geon_sched.hpp")] + \
+                            for f in ("geon_device.hpp", "geon_fp32.hpp", "geon_sched.hpp",
+                                      "geon_curv.hpp")] + \
     [os.path.join(CSRC, "geon_dev_types.h")]
 HOST_SRC = [os.path.join(CSRC, "host", f)
             for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp", "geon_serve.cpp")]

@@ -260,15 +260,19 @@ int geon_curvature_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const doubl
     hipSetDevice(ctx->device);
     geon_dframe F;
     GEON_TRY(fill_frame(metric, &F));
-    void* scratch = nullptr;
-    if (what == GEON_CURV_RIEMANN) GEON_TRY(ctx_buffer(ctx, "curv_scratch", n * 256 * 8, &scratch));
     hipStream_t s = pick_stream(ctx, stream);
     uint32_t slot;
     GEON_TRY(ctx_slot(ctx, &F, s, &slot));
     uint64_t nn = n;
     double factor = 1.0 / (8.0 * kPi);  // curvature.rs:118
+    void* scratch = nullptr;
+    if (what == GEON_CURV_RIEMANN && metric->kind != GEON_METRIC_ALCUBIERRE)
+        GEON_TRY(ctx_buffer(ctx, "curv_scratch", n * 256 * 8, &scratch));
     void* args[] = {&slot, &nn, (void*)&x_soa, &what, &out, &scratch, &factor};
-    GEON_TRY(launch(ctx->fn_curv[metric->kind], nn, 256, s, args));
+    if (metric->kind == GEON_METRIC_ALCUBIERRE)   // one 8-lane team per point, 8 points per block
+        GEON_TRY(launch(ctx->fn_curv[metric->kind], ((nn + 7) / 8) * 64, 64, s, args));
+    else                                          // one lane per point
+        GEON_TRY(launch(ctx->fn_curv[metric->kind], nn, 256, s, args));
     return ctx_slot_done(ctx, slot, s);
 }
 

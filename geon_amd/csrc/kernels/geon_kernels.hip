@@ -7,6 +7,7 @@
 // of an n-ray array starts at base + k·n, so consecutive lanes touch
 // consecutive addresses.
 #include "geon_device.hpp"
+#include "geon_curv.hpp"
 #include "geon_sched.hpp"
 
 using namespace geon;
@@ -370,10 +371,22 @@ __device__ void curvature_point(uint32_t slot, uint64_t n, const double* __restr
         curvature_point<MP>(slot, n, x, what, out, scratch, factor);                            \
     }
 
+// Metrics with analytic or cheap Γ keep the whole point in one lane.
 GEON_CURV_KERNEL(mink, Minkowski)
 GEON_CURV_KERNEL(schw, Schwarzschild)
-GEON_CURV_KERNEL(alc, Alcubierre)
 GEON_CURV_KERNEL(mt, MorrisThorne)
+
+// Alcubierre's Γ is expensive (six shifted metrics, each with a square root
+// and two tanh): its seven stencil evaluations run on seven lanes of a team
+// and meet in LDS (geon_curv.hpp); 64-lane blocks, 8 points per block.
+// Same outputs bit for bit as the one-lane form, without its register
+// spills.
+extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc(
+    uint32_t slot, uint64_t n, const double* x, int32_t what, double* out, double* scratch,
+    double factor) {
+    (void)scratch;
+    curvature_team<Alcubierre>(c_frames[slot].m, n, x, what, out, factor);
+}
 
 // ---- shadow bisection (validate_shadow.rs:20-95) ----------------------------
 // One lane per problem k: Schwarzschild::new(mass[k]) (pad 1.001, escape
