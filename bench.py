@@ -406,7 +406,11 @@ def curvature(geon, ctx, torch, stream, ev, alc, mt):
 
 
 def cpu_baseline(jobs):
-    """Oracle (reference algorithm in C) on a bounded sample of the Morris–Thorne rays."""
+    """Oracle (reference algorithm in C) on a bounded sample of the Morris–Thorne rays:
+    1024 random pixels on all the threads it may use (~25 CPU-seconds), and 64
+    of them on one thread."""
+    import platform
+
     import numpy as np
 
     from oracle import oracle as O
@@ -415,16 +419,28 @@ def cpu_baseline(jobs):
     ocam = O.camera(cam.observer, cam.fov_y_deg, cam.width, cam.height)
     st, tet = O.tetrad(om, ocam.observer)
     rng = np.random.default_rng(0)
-    idx = rng.choice(cam.width * cam.height, size=256, replace=False)
+    idx = rng.choice(cam.width * cam.height, size=1024, replace=False)
     y0 = np.array([O.ray_state_sub(ocam, tet, int(i) % cam.width, int(i) // cam.width)
                    for i in idx])
     threads = O.THREADS
     res = O.trace_rays(om, y0, cfg.step, cfg.max_steps, nthreads=threads)
     steps = int(res["steps"].sum())
+    one = O.trace_rays(om, y0[:64], cfg.step, cfg.max_steps, nthreads=1)
+    model = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": steps / res["seconds"], "unit": "ray-steps/s", "cores": threads,
             "kind": "port",
             "sample": f"{len(idx)} random pixels of the Morris-Thorne 1024² render "
-                      f"({steps} ray-steps, {res['seconds']:.1f} s)"}
+                      f"({steps} ray-steps, {res['seconds']:.2f} s on {threads} threads)",
+            "single_thread_ray_steps_per_s": int(one["steps"].sum()) / one["seconds"],
+            "host_cpus": os.cpu_count(), "cpu_model": model}
 
 
 if __name__ == "__main__":
