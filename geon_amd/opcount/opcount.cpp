@@ -1,8 +1,9 @@
-// opcount.cpp — counts the FP64 operations of the device formulation.
+// opcount.cpp — counts the operations of the device formulation.
 //
-// The gfx950 kernels' per-ray code (geon_device.hpp) is compiled for the
-// host with every `double` replaced by a counting scalar, then one RK4 step
-// per metric is evaluated. The tallies are the algorithmic operation counts of exactly
+// The gfx950 kernels' per-ray code (geon_device.hpp, FP64; geon_fp32.hpp,
+// the FP32 side of the precision split) is compiled for the host with every
+// `double` and `float` replaced by counting scalars, then one RK4 step per
+// metric is evaluated. The tallies are the algorithmic operation counts of exactly
 // the arithmetic the kernels execute (structural zeros already removed):
 // add, sub, mul, div, sqrt, sin, cos, tanh count 1 each (sincos counts 2,
 // an FMA counts as a mul and an add).
@@ -90,6 +91,55 @@ inline cnt_d div_rn_pz(cnt_d a, cnt_d b, cnt_d) { ++T.div; return cnt_d(a.v / b.
 #undef __builtin_fma
 #undef __builtin_isnan
 
+// ---- FP32 side: the same for every `float` of geon_fp32.hpp --------------
+// The reciprocal builtin counts as one division, __expf as one
+// transcendental (tallied with tanh), sqrtf as sqrt.
+struct cnt_f {
+    float v;
+    cnt_f() = default;
+    cnt_f(float x) : v(x) {}
+    explicit cnt_f(cnt_d x) : v((float)x.v) {}
+    explicit operator float() const { return v; }
+    cnt_f& operator+=(cnt_f o) { ++T.add; v += o.v; return *this; }
+    cnt_f& operator-=(cnt_f o) { ++T.sub; v -= o.v; return *this; }
+    cnt_f& operator*=(cnt_f o) { ++T.mul; v *= o.v; return *this; }
+};
+inline cnt_f operator+(cnt_f a, cnt_f b) { ++T.add; return cnt_f(a.v + b.v); }
+inline cnt_f operator-(cnt_f a, cnt_f b) { ++T.sub; return cnt_f(a.v - b.v); }
+inline cnt_f operator*(cnt_f a, cnt_f b) { ++T.mul; return cnt_f(a.v * b.v); }
+inline cnt_f operator/(cnt_f a, cnt_f b) { ++T.div; return cnt_f(a.v / b.v); }
+inline cnt_f operator-(cnt_f a) { ++T.neg; return cnt_f(-a.v); }
+inline bool operator<(cnt_f a, cnt_f b) { ++T.cmp; return a.v < b.v; }
+inline bool operator>(cnt_f a, cnt_f b) { ++T.cmp; return a.v > b.v; }
+inline bool operator<=(cnt_f a, cnt_f b) { ++T.cmp; return a.v <= b.v; }
+inline bool operator>=(cnt_f a, cnt_f b) { ++T.cmp; return a.v >= b.v; }
+inline bool operator==(cnt_f a, cnt_f b) { ++T.cmp; return a.v == b.v; }
+inline cnt_f fabsf(cnt_f a) { ++T.fabs; return cnt_f(::fabsf(a.v)); }
+inline cnt_f fmaxf(cnt_f a, cnt_f b) { ++T.fmax; return cnt_f(::fmaxf(a.v, b.v)); }
+inline cnt_f fminf(cnt_f a, cnt_f b) { ++T.fmax; return cnt_f(::fminf(a.v, b.v)); }
+inline void sincosf(cnt_f a, cnt_f* s, cnt_f* c) {
+    ++T.sin;
+    ++T.cos;
+    *s = cnt_f(::sinf(a.v));
+    *c = cnt_f(::cosf(a.v));
+}
+inline cnt_f cnt_rcpf(cnt_f a) { ++T.div; return cnt_f(1.0f / a.v); }
+inline cnt_f cnt_sqrtf(cnt_f a) { ++T.sqrt; return cnt_f(::sqrtf(a.v)); }
+inline cnt_f cnt_expf(cnt_f a) { ++T.tanh; return cnt_f(::expf(a.v)); }
+inline bool cnt_isfinite(cnt_f a) { return ::isfinite(a.v); }
+
+#define __builtin_amdgcn_rcpf(x) cnt_rcpf(x)
+#define __builtin_sqrtf(x) cnt_sqrtf(x)
+#define __expf(x) cnt_expf(x)
+#define __builtin_isfinite(x) cnt_isfinite(x)
+#define float cnt_f
+#include "../csrc/kernels/geon_fp32.hpp"
+#undef float
+#undef __builtin_amdgcn_rcpf
+#undef __builtin_sqrtf
+#undef __expf
+#undef __builtin_isfinite
+
 using namespace geon;
 
 static geon_dmetric metric(int kind) {
@@ -127,6 +177,17 @@ static OpTally step_ops(const geon_dmetric& P, const double* y0) {
     return T;
 }
 
+template <class MF>
+static OpTally step_ops_f(const geon_dmetric& Pd, const double* y0) {
+    const FParams P = fparams(Pd);
+    cnt_f y[8], out[8];
+    for (int i = 0; i < 8; ++i) y[i] = cnt_f((float)y0[i]);
+    cnt_f s = cnt_f(::sinf((float)y0[2])), c = cnt_f(::cosf((float)y0[2]));
+    memset(&T, 0, sizeof T);
+    rk4_step_f<MF>(P, y, cnt_f(0.05f), out, s, c);
+    return T;
+}
+
 static void emit(const char* name, const OpTally& t, bool last) {
     printf("    \"%s\": {\"ops\": %llu, \"add\": %llu, \"sub\": %llu, \"mul\": %llu, \"div\": %llu, "
            "\"sqrt\": %llu, \"sin\": %llu, \"cos\": %llu, \"tanh\": %llu, "
@@ -150,6 +211,12 @@ int main() {
     emit("morris_thorne", step_ops<MorrisThorne>(metric(3), ym), false);
     emit("alcubierre_wall", step_ops<Alcubierre>(metric(2), ya_wall), false);
     emit("alcubierre_flat", step_ops<Alcubierre>(metric(2), ya_flat), true);
+    printf("  },\n  \"ray_step_fp32\": {\n");
+    emit("minkowski", step_ops_f<MinkowskiF>(metric(0), yk), false);
+    emit("schwarzschild", step_ops_f<SchwarzschildF>(metric(1), ys), false);
+    emit("morris_thorne", step_ops_f<MorrisThorneF>(metric(3), ym), false);
+    emit("alcubierre_wall", step_ops_f<AlcubierreF>(metric(2), ya_wall), false);
+    emit("alcubierre_flat", step_ops_f<AlcubierreF>(metric(2), ya_flat), true);
     printf("  }\n}\n");
     return 0;
 }
