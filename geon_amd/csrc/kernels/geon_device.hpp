@@ -42,7 +42,7 @@ GEON_D void sfor(F&& f) {
 // the quotient are normal numbers far from overflow, which each caller
 // guarantees or guards. Saves the ~11-instruction IEEE division sequence
 // where one reciprocal serves several quotients or is a constant.
-#ifndef GEON_OPCOUNT_DIV_RN   // the op-counting build tallies these as one division
+#ifndef GEON_OPCOUNT   // the op-counting build supplies these, tallied as one division each
 GEON_D double div_rn(double a, double b, double y) {
     const double q = a * y;
     const double r = __builtin_fma(-q, b, a);
@@ -53,6 +53,50 @@ GEON_D double div_rn(double a, double b, double y) {
 GEON_D double div_rn_pz(double a, double b, double y) {
     const double q = a * y;
     return __builtin_fma(__builtin_fma(-q, b, a), y, q);
+}
+#endif
+
+// sin x and cos x together, for |x| < 2^20 (else the library sincos): the
+// Cody–Waite reduction x = k·π/2 + r with π/2 split into two doubles and
+// both products taken exactly inside FMAs, then Taylor polynomials of sin to
+// r^17 and of cos to r^18 on |r| ≤ π/4 (truncation below 1e-19 relative)
+// and the quadrant k mod 4. Within 1.5 ulp of the correctly rounded sine and
+// cosine (the reduced argument's own rounding and the final sums; checked
+// against 40-digit arithmetic by the test suite) at about half the
+// instructions of the general library routine.
+#ifndef GEON_OPCOUNT   // the op-counting build tallies this as one sine and one cosine
+GEON_D void sincos_cw(double x, double* s, double* c) {
+    if (!(fabs(x) < 0x1p20)) {
+        sincos(x, s, c);
+        return;
+    }
+    const double k = rint(x * 0x1.45f306dc9c883p-1);          // x·(2/π)
+    double r = __builtin_fma(-k, 0x1.921fb54442d18p+0, x);    // − k·hi(π/2)
+    r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);           // − k·lo(π/2)
+    const double r2 = r * r;
+    double ps = 0x1.952c77030ad4ap-49;                         //  1/17!
+    ps = __builtin_fma(ps, r2, -0x1.ae7f3e733b81fp-41);        // −1/15!
+    ps = __builtin_fma(ps, r2, 0x1.6124613a86d09p-33);         //  1/13!
+    ps = __builtin_fma(ps, r2, -0x1.ae64567f544e4p-26);        // −1/11!
+    ps = __builtin_fma(ps, r2, 0x1.71de3a556c734p-19);         //  1/9!
+    ps = __builtin_fma(ps, r2, -0x1.a01a01a01a01ap-13);        // −1/7!
+    ps = __builtin_fma(ps, r2, 0x1.1111111111111p-7);          //  1/5!
+    ps = __builtin_fma(ps, r2, -0x1.5555555555555p-3);         // −1/3!
+    const double sr = __builtin_fma(r * r2, ps, r);
+    double pc = -0x1.6827863b97d97p-53;                        // −1/18!
+    pc = __builtin_fma(pc, r2, 0x1.ae7f3e733b81fp-45);         //  1/16!
+    pc = __builtin_fma(pc, r2, -0x1.93974a8c07c9dp-37);        // −1/14!
+    pc = __builtin_fma(pc, r2, 0x1.1eed8eff8d898p-29);         //  1/12!
+    pc = __builtin_fma(pc, r2, -0x1.27e4fb7789f5cp-22);        // −1/10!
+    pc = __builtin_fma(pc, r2, 0x1.a01a01a01a01ap-16);         //  1/8!
+    pc = __builtin_fma(pc, r2, -0x1.6c16c16c16c17p-10);        // −1/6!
+    pc = __builtin_fma(pc, r2, 0x1.5555555555555p-5);          //  1/4!
+    const double cr = __builtin_fma(r2 * r2, pc, __builtin_fma(-0.5, r2, 1.0));
+    const int q = (int)k & 3;
+    const double sv = (q & 1) ? cr : sr;
+    const double cv = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -sv : sv;
+    *c = ((q + 1) & 2) ? -cv : cv;
 }
 #endif
 
@@ -351,8 +395,9 @@ struct MorrisThorne {
     // The generic Γ of metric/mod.rs:51-69 with ∂g by central differences
     // (metric/mod.rs:25-43) written out for this metric: g_θθ = b0² + l² and
     // g_φφ = g_θθ sin²θ, so the centre and the l ± h evaluations share
-    // sin θ and b0². sin(θ ± h) comes from one sincos θ rotated by the
-    // constants sin h, cos h (angle addition) instead of two more sines:
+    // sin θ and b0². sin θ and cos θ come from sincos_cw; sin(θ ± h) from
+    // rotating them by the constants sin h, cos h (angle addition) instead
+    // of two more sines:
     // each differs from a directly evaluated sine by at most an ulp, the
     // same size as one libm against another, so ∂_θ g_φφ moves by
     // ~1e-11 relative (its FD quotient amplifies by 1/h). Every other entry
@@ -361,7 +406,7 @@ struct MorrisThorne {
         const double l = x[1], th = x[2], h = P.fd_h, two_h = P.fd_two_h;
         const double b2 = P.b0 * P.b0;
         double st, ct;
-        sincos(th, &st, &ct);
+        sincos_cw(th, &st, &ct);
         const double r2 = b2 + l * l;
         double g0[16], gi[16], dgv[64];
         g0[0] = -1.0; g0[5] = 1.0; g0[10] = r2;

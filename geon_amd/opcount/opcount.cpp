@@ -80,7 +80,9 @@ inline bool cnt_isnan(cnt_d a) { return ::isnan(a.v); }
 // a quotient from a precomputed reciprocal is one algorithmic division
 inline cnt_d div_rn(cnt_d a, cnt_d b, cnt_d) { ++T.div; return cnt_d(a.v / b.v); }
 inline cnt_d div_rn_pz(cnt_d a, cnt_d b, cnt_d) { ++T.div; return cnt_d(a.v / b.v); }
-#define GEON_OPCOUNT_DIV_RN 1
+// the kernels' own sincos is one sine and one cosine
+inline void sincos_cw(cnt_d a, cnt_d* s, cnt_d* c) { sincos(a, s, c); }
+#define GEON_OPCOUNT 1
 #define __builtin_isfinite(x) cnt_isfinite(x)
 #define __builtin_fma(a, b, c) cnt_fma(a, b, c)
 #define __builtin_isnan(x) cnt_isnan(x)
