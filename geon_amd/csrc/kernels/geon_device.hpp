@@ -56,6 +56,17 @@ GEON_D double div_rn_pz(double a, double b, double y) {
 }
 #endif
 
+// A constant pinned to an SGPR pair where it is used. Loop-invariant FP64
+// constants otherwise end up in VGPRs, and an FMA adding one becomes the
+// two-address v_fmac behind a v_mov_b64 copy of the constant; as an SGPR
+// operand it is a plain three-address v_fma_f64. The value is unchanged.
+GEON_D double sconst(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("" : "+s"(c));
+#endif
+    return c;
+}
+
 // sin x and cos x together, for |x| < 2^20 (else the library sincos): the
 // Cody–Waite reduction x = k·π/2 + r with π/2 split into two doubles and
 // both products taken exactly inside FMAs, then Taylor polynomials of sin to
@@ -75,22 +86,22 @@ GEON_D void sincos_cw(double x, double* s, double* c) {
     r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r);           // − k·lo(π/2)
     const double r2 = r * r;
     double ps = 0x1.952c77030ad4ap-49;                         //  1/17!
-    ps = __builtin_fma(ps, r2, -0x1.ae7f3e733b81fp-41);        // −1/15!
-    ps = __builtin_fma(ps, r2, 0x1.6124613a86d09p-33);         //  1/13!
-    ps = __builtin_fma(ps, r2, -0x1.ae64567f544e4p-26);        // −1/11!
-    ps = __builtin_fma(ps, r2, 0x1.71de3a556c734p-19);         //  1/9!
-    ps = __builtin_fma(ps, r2, -0x1.a01a01a01a01ap-13);        // −1/7!
-    ps = __builtin_fma(ps, r2, 0x1.1111111111111p-7);          //  1/5!
-    ps = __builtin_fma(ps, r2, -0x1.5555555555555p-3);         // −1/3!
+    ps = __builtin_fma(ps, r2, sconst(-0x1.ae7f3e733b81fp-41));     // −1/15!
+    ps = __builtin_fma(ps, r2, sconst(0x1.6124613a86d09p-33));      //  1/13!
+    ps = __builtin_fma(ps, r2, sconst(-0x1.ae64567f544e4p-26));     // −1/11!
+    ps = __builtin_fma(ps, r2, sconst(0x1.71de3a556c734p-19));      //  1/9!
+    ps = __builtin_fma(ps, r2, sconst(-0x1.a01a01a01a01ap-13));     // −1/7!
+    ps = __builtin_fma(ps, r2, sconst(0x1.1111111111111p-7));       //  1/5!
+    ps = __builtin_fma(ps, r2, sconst(-0x1.5555555555555p-3));      // −1/3!
     const double sr = __builtin_fma(r * r2, ps, r);
     double pc = -0x1.6827863b97d97p-53;                        // −1/18!
-    pc = __builtin_fma(pc, r2, 0x1.ae7f3e733b81fp-45);         //  1/16!
-    pc = __builtin_fma(pc, r2, -0x1.93974a8c07c9dp-37);        // −1/14!
-    pc = __builtin_fma(pc, r2, 0x1.1eed8eff8d898p-29);         //  1/12!
-    pc = __builtin_fma(pc, r2, -0x1.27e4fb7789f5cp-22);        // −1/10!
-    pc = __builtin_fma(pc, r2, 0x1.a01a01a01a01ap-16);         //  1/8!
-    pc = __builtin_fma(pc, r2, -0x1.6c16c16c16c17p-10);        // −1/6!
-    pc = __builtin_fma(pc, r2, 0x1.5555555555555p-5);          //  1/4!
+    pc = __builtin_fma(pc, r2, sconst(0x1.ae7f3e733b81fp-45));      //  1/16!
+    pc = __builtin_fma(pc, r2, sconst(-0x1.93974a8c07c9dp-37));     // −1/14!
+    pc = __builtin_fma(pc, r2, sconst(0x1.1eed8eff8d898p-29));      //  1/12!
+    pc = __builtin_fma(pc, r2, sconst(-0x1.27e4fb7789f5cp-22));     // −1/10!
+    pc = __builtin_fma(pc, r2, sconst(0x1.a01a01a01a01ap-16));      //  1/8!
+    pc = __builtin_fma(pc, r2, sconst(-0x1.6c16c16c16c17p-10));     // −1/6!
+    pc = __builtin_fma(pc, r2, sconst(0x1.5555555555555p-5));       //  1/4!
     const double cr = __builtin_fma(r2 * r2, pc, __builtin_fma(-0.5, r2, 1.0));
     const int q = (int)k & 3;
     const double sv = (q & 1) ? cr : sr;
