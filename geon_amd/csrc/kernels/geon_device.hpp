@@ -154,6 +154,23 @@ struct FD {
         else return -vc;
     }
 
+    // ∂_a g from the metric at x + h e_a (gp) and x − h e_a (gm)
+    template <int a>
+    GEON_D static void dg_dir(const geon_dmetric& P, const double* gp, const double* gm,
+                              double* dg) {
+        sfor<4>([&](auto Mu) {
+            sfor<4>([&](auto Nu) {
+                constexpr int mu = CI(Mu), nu = CI(Nu);
+                if constexpr (MP::dg_nz(a, mu, nu)) {
+                    if constexpr (nu < mu && MP::dg_nz(a, nu, mu))
+                        dg[a * 16 + mu * 4 + nu] = dg[a * 16 + nu * 4 + mu];  // symmetric g
+                    else
+                        dg[a * 16 + mu * 4 + nu] = (gp[mu * 4 + nu] - gm[mu * 4 + nu]) / P.fd_two_h;
+                }
+            });
+        });
+    }
+
     // ∂g by central differences over the coordinates g depends on.
     GEON_D static void dg(const geon_dmetric& P, const double* x, double* dg) {
         sfor<4>([&](auto A) {
@@ -166,18 +183,7 @@ struct FD {
                 double gp[16], gm[16];
                 MP::g(P, xp, gp);
                 MP::g(P, xm, gm);
-                sfor<4>([&](auto Mu) {
-                    sfor<4>([&](auto Nu) {
-                        constexpr int mu = CI(Mu), nu = CI(Nu);
-                        if constexpr (MP::dg_nz(a, mu, nu)) {
-                            if constexpr (nu < mu && MP::dg_nz(a, nu, mu))
-                                dg[a * 16 + mu * 4 + nu] = dg[a * 16 + nu * 4 + mu];  // symmetric g
-                            else
-                                dg[a * 16 + mu * 4 + nu] =
-                                    (gp[mu * 4 + nu] - gm[mu * 4 + nu]) / P.fd_two_h;
-                        }
-                    });
-                });
+                dg_dir<a>(P, gp, gm, dg);
             }
         });
     }
@@ -483,13 +489,16 @@ struct Alcubierre {
         }
         return (tp - tm) / P.denom;
     }
-    GEON_D static void g(const geon_dmetric& P, const double* x, double* g) {
+    // g from the shape value f at the point (alcubierre.rs:48-63)
+    GEON_D static void g_of_f(const geon_dmetric& P, double f, double* g) {
         const double v = P.v;
-        const double f = shape(P, rs(P, x));
         g[0] = -1.0 + v * v * f * f;
         g[1] = -v * f;
         g[4] = -v * f;
         g[5] = 1.0; g[10] = 1.0; g[15] = 1.0;
+    }
+    GEON_D static void g(const geon_dmetric& P, const double* x, double* g) {
+        g_of_f(P, shape(P, rs(P, x)), g);
     }
     // Cofactor inverse of [[a,b],[b,1]] ⊕ I: det = a − b², g^00 = 1/det,
     // g^01 = g^10 = −b/det, g^11 = a/det, g^22 = g^33 = det/det = 1.

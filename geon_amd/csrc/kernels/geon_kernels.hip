@@ -376,16 +376,70 @@ GEON_CURV_KERNEL(mink, Minkowski)
 GEON_CURV_KERNEL(schw, Schwarzschild)
 GEON_CURV_KERNEL(mt, MorrisThorne)
 
-// Alcubierre's Γ is expensive (six shifted metrics, each with a square root
-// and two tanh): its seven stencil evaluations run on seven lanes of a team
-// and meet in LDS (geon_curv.hpp); 64-lane blocks, 8 points per block.
-// Same outputs bit for bit as the one-lane form, without its register
-// spills.
+// Alcubierre's Γ is expensive (seven metric evaluations, each a square root
+// and two tanh), and the seven Γ of a point's stencil share most of their
+// metric points: a team of 8 lanes per point evaluates the 31 distinct
+// metric points once, forms the seven Γ on seven lanes and the outputs on
+// all eight, exchanging through LDS (geon_curv.hpp); 64-lane blocks,
+// 8 points per block. Same outputs bit for bit as one lane per point.
 extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc(
-    uint32_t slot, uint64_t n, const double* x, int32_t what, double* out, double* scratch,
+    uint32_t slot, uint64_t n, const double* xs, int32_t what, double* out, double* scratch,
     double factor) {
     (void)scratch;
-    curvature_team<Alcubierre>(c_frames[slot].m, n, x, what, out, factor);
+    using CS = CurvStencil<Alcubierre>;
+    static_assert(CS::kPoints <= GEON_CURV_TEAM, "stencil larger than the team");
+    __shared__ double s_f[GEON_CURV_PTS][AlcStencil::kRow];
+    __shared__ double s_gam[GEON_CURV_PTS][CS::kRow];
+    __shared__ double s_ric[GEON_CURV_PTS][17];
+    const geon_dmetric& P = c_frames[slot].m;
+    const int t = (int)(threadIdx.x % GEON_CURV_TEAM);
+    const int q = (int)(threadIdx.x / GEON_CURV_TEAM);
+    const uint64_t k = (uint64_t)blockIdx.x * GEON_CURV_PTS + q;
+    const bool valid = k < n;
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+        x[0] = xs[k];
+        x[1] = xs[n + k];
+        x[2] = xs[2 * n + k];
+        x[3] = xs[3 * n + k];
+    }
+    // A: the 31 distinct metric points, 4 rounds of the 8 lanes
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int j = t + GEON_CURV_TEAM * r;
+        if (valid && j < AlcStencil::kUsed) s_f[q][j] = curv_alc_point_f(P, x, j);
+    }
+    __syncthreads();
+    // B: Γ at the seven stencil points
+    if (valid && t < CS::kPoints) curv_alc_gamma(P, s_f[q], t, &s_gam[q][t * CS::kNNZ]);
+    __syncthreads();
+    // C: outputs
+    if (what == 0) {
+        if (!valid) return;
+        curv_riemann_lane<Alcubierre>(P, s_gam[q], t, out + k * 256);
+        return;
+    }
+    double ric[2];
+    curv_ricci_lane<Alcubierre>(P, s_gam[q], t, ric);
+    if (what == 1) {
+        if (valid) {
+            out[k * 16 + t] = ric[0];
+            out[k * 16 + t + GEON_CURV_TEAM] = ric[1];
+        }
+        return;
+    }
+    s_ric[q][t] = ric[0];
+    s_ric[q][t + GEON_CURV_TEAM] = ric[1];
+    __syncthreads();   // all 16 components of every point are in LDS
+    if (!valid) return;
+    double e[2];
+    curv_alc_outputs_lane(P, s_f[q][0], s_ric[q], t, what, factor, e);
+    if (what == 4) {
+        if (t == 0) out[k] = e[0];
+        return;
+    }
+    out[k * 16 + t] = e[0];
+    out[k * 16 + t + GEON_CURV_TEAM] = e[1];
 }
 
 // ---- shadow bisection (validate_shadow.rs:20-95) ----------------------------

@@ -190,6 +190,76 @@ static OpTally step_ops_f(const geon_dmetric& Pd, const double* y0) {
     return T;
 }
 
+// ---- curvature per point (curvature.rs:25-141) ----------------------------
+// The formulation of the curvature kernels for one point with every
+// structural zero left out: Γ at the point and at x ± h·e_d for each
+// coordinate d that Γ depends on (the policies' own christoffel), the ∂Γ
+// entries Ricci reads, each Ricci component as one signed sum of its nonzero
+// terms, R = Σ g^μν R_μν, G = R_μν − ½ g_μν R and T = G/(8π) (what = 2 stops
+// at G). A sum of k terms is k − 1 additions; terms that are structurally
+// zero cost nothing.
+struct SAcc {
+    cnt_d v = cnt_d(0.0);
+    bool any = false;
+    void add(cnt_d t) { v = any ? v + t : t; any = true; }
+    void sub(cnt_d t) { v = any ? v - t : -t; any = true; }
+};
+
+template <class MP>
+static OpTally curvature_ops(const geon_dmetric& P, const double* x0, int what) {
+    memset(&T, 0, sizeof T);
+    cnt_d x[4];
+    for (int i = 0; i < 4; ++i) x[i] = x0[i];
+    cnt_d gam[64], D1[64], D2[64];
+    MP::christoffel(P, x, gam);
+    for (int d = 0; d < 4; ++d) {
+        if (!MP::dep(d)) continue;
+        cnt_d xp[4], xm[4], gp[64], gm[64];
+        for (int i = 0; i < 4; ++i) xp[i] = xm[i] = x[i];
+        xp[d] += cnt_d(P.fd_h);
+        xm[d] -= cnt_d(P.fd_h);
+        MP::christoffel(P, xp, gp);
+        MP::christoffel(P, xm, gm);
+        for (int b = 0; b < 4; ++b)
+            for (int c = 0; c < 4; ++c) {
+                const int i1 = d * 16 + b * 4 + c, i2 = b * 16 + b * 4 + c;
+                if (MP::gam_nz(d, b, c)) D1[d * 16 + b * 4 + c] = (gp[i1] - gm[i1]) * cnt_d(P.fd_inv_2h);
+                if (MP::gam_nz(b, b, c)) D2[d * 16 + b * 4 + c] = (gp[i2] - gm[i2]) * cnt_d(P.fd_inv_2h);
+            }
+    }
+    cnt_d ric[16];
+    bool ric_nz[16];
+    for (int sg = 0; sg < 4; ++sg)
+        for (int nu = 0; nu < 4; ++nu) {
+            SAcc s;
+            for (int a = 0; a < 4; ++a) {
+                if (MP::dep(a) && MP::gam_nz(a, nu, sg)) s.add(D1[a * 16 + nu * 4 + sg]);
+                if (MP::dep(nu) && MP::gam_nz(a, a, sg)) s.sub(D2[nu * 16 + a * 4 + sg]);
+                for (int l = 0; l < 4; ++l) {
+                    if (MP::gam_nz(a, a, l) && MP::gam_nz(l, nu, sg))
+                        s.add(gam[a * 16 + a * 4 + l] * gam[l * 16 + nu * 4 + sg]);
+                    if (MP::gam_nz(a, nu, l) && MP::gam_nz(l, a, sg))
+                        s.sub(gam[a * 16 + nu * 4 + l] * gam[l * 16 + a * 4 + sg]);
+                }
+            }
+            ric[sg * 4 + nu] = s.v;
+            ric_nz[sg * 4 + nu] = s.any;
+        }
+    cnt_d gi[16], g[16];
+    MP::ginv_at(P, x, gi);
+    SAcc R;
+    for (int i = 0; i < 16; ++i)
+        if (MP::ginv_nz(i / 4, i % 4) && ric_nz[i]) R.add(gi[i] * ric[i]);
+    MP::g(P, x, g);
+    for (int i = 0; i < 16; ++i) {
+        SAcc e;
+        if (ric_nz[i]) e.add(ric[i]);
+        if (MP::g_nz(i / 4, i % 4) && R.any) e.sub(cnt_d(0.5) * g[i] * R.v);
+        if (e.any && what == 3) e.v = e.v * cnt_d(1.0 / (8.0 * M_PI));
+    }
+    return T;
+}
+
 static void emit(const char* name, const OpTally& t, bool last) {
     printf("    \"%s\": {\"ops\": %llu, \"add\": %llu, \"sub\": %llu, \"mul\": %llu, \"div\": %llu, "
            "\"sqrt\": %llu, \"sin\": %llu, \"cos\": %llu, \"tanh\": %llu, "
@@ -206,7 +276,7 @@ int main() {
     const double ya_wall[8] = {0.0, 2.0, 0.3, 0.2, 1.2, -0.5, 0.3, 0.1};
     const double ya_flat[8] = {0.0, 8.0, 0.3, 0.2, 1.2, -0.5, 0.3, 0.1};
     const double yk[8] = {0.0, 5.0, 1.0, 2.0, 1.0, -0.6, 0.0, 0.8};
-    printf("{\n  \"unit\": \"FP64 operations per RK4 ray-step (rk4_step of geon_device.hpp)\",\n");
+    printf("{\n  \"unit\": \"FP64 operations per RK4 ray-step (rk4_step of geon_device.hpp) and per curvature point\",\n");
     printf("  \"ray_step\": {\n");
     emit("minkowski", step_ops<Minkowski>(metric(0), yk), false);
     emit("schwarzschild", step_ops<Schwarzschild>(metric(1), ys), false);
@@ -219,6 +289,18 @@ int main() {
     emit("morris_thorne", step_ops_f<MorrisThorneF>(metric(3), ym), false);
     emit("alcubierre_wall", step_ops_f<AlcubierreF>(metric(2), ya_wall), false);
     emit("alcubierre_flat", step_ops_f<AlcubierreF>(metric(2), ya_flat), true);
+    // curvature sample points: no branch of any Γ depends on the point here
+    const double xs[4] = {0.0, 10.0, 1.2, 0.3};
+    const double xa[4] = {0.0, 1.9, 0.4, -0.3};
+    const double xm[4] = {0.0, 0.7, 1.5707963267948966, 0.0};
+    const double xk[4] = {0.0, 1.0, 2.0, 3.0};
+    for (int what = 2; what <= 3; ++what) {
+        printf("  },\n  \"%s\": {\n", what == 2 ? "curvature_einstein" : "curvature_stress");
+        emit("minkowski", curvature_ops<Minkowski>(metric(0), xk, what), false);
+        emit("schwarzschild", curvature_ops<Schwarzschild>(metric(1), xs, what), false);
+        emit("morris_thorne", curvature_ops<MorrisThorne>(metric(3), xm, what), false);
+        emit("alcubierre", curvature_ops<Alcubierre>(metric(2), xa, what), true);
+    }
     printf("  }\n}\n");
     return 0;
 }

@@ -1,0 +1,81 @@
+// Host build of the kernels' FP64 device code (geon_device.hpp,
+// geon_curv.hpp) for tests/test_device_host.py: traces, Γ, and the
+// Alcubierre curvature team run lane after lane with its LDS rows in
+// ordinary arrays. Metric constants come from the runtime's own
+// make_dmetric.
+#include <string.h>
+
+#include "../../geon_amd/csrc/host/geon_runtime.hpp"
+#include "../../geon_amd/csrc/kernels/geon_curv.hpp"
+
+using namespace geon;
+using geon_host::make_dmetric;
+
+extern "C" int hd_trace(const geon_metric_t* m, double* y, double h, uint32_t max_steps,
+                        uint32_t* steps) {
+    geon_dmetric P;
+    if (make_dmetric(m, &P) != GEON_OK) return -100;
+    switch (m->kind) {
+        case GEON_METRIC_MINKOWSKI: return trace<Minkowski>(P, y, h, max_steps, steps);
+        case GEON_METRIC_SCHWARZSCHILD: return trace<Schwarzschild>(P, y, h, max_steps, steps);
+        case GEON_METRIC_ALCUBIERRE: return trace<Alcubierre>(P, y, h, max_steps, steps);
+        default: return trace<MorrisThorne>(P, y, h, max_steps, steps);
+    }
+}
+
+extern "C" int hd_christoffel(const geon_metric_t* m, const double* x, double* c) {
+    geon_dmetric P;
+    if (make_dmetric(m, &P) != GEON_OK) return -100;
+    memset(c, 0, 64 * sizeof(double));
+    switch (m->kind) {
+        case GEON_METRIC_SCHWARZSCHILD: Schwarzschild::christoffel(P, x, c); break;
+        case GEON_METRIC_ALCUBIERRE: Alcubierre::christoffel(P, x, c); break;
+        case GEON_METRIC_MORRIS_THORNE: MorrisThorne::christoffel(P, x, c); break;
+        default: break;
+    }
+    return 0;
+}
+
+// what: 0 Riemann (256 per point), 1 Ricci, 2 Einstein, 3 stress-energy
+// (16 per point), 4 Ricci scalar (1 per point); x is n×4.
+extern "C" int hd_curvature_alc_team(const geon_metric_t* m, const double* x, long n, int what,
+                                     double factor, double* out) {
+    using CS = CurvStencil<Alcubierre>;
+    geon_dmetric P;
+    if (m->kind != GEON_METRIC_ALCUBIERRE || make_dmetric(m, &P) != GEON_OK) return -100;
+    for (long k = 0; k < n; ++k) {
+        const double* xk = x + 4 * k;
+        double sf[AlcStencil::kRow], row[CS::kRow], ric[17];
+        for (int t = 0; t < GEON_CURV_TEAM; ++t)   // phase A
+            for (int r = 0; r < 4; ++r) {
+                const int j = t + GEON_CURV_TEAM * r;
+                if (j < AlcStencil::kUsed) sf[j] = curv_alc_point_f(P, xk, j);
+            }
+        for (int t = 0; t < CS::kPoints; ++t) curv_alc_gamma(P, sf, t, &row[t * CS::kNNZ]);
+        if (what == 0) {
+            for (int t = 0; t < GEON_CURV_TEAM; ++t) curv_riemann_lane<Alcubierre>(P, row, t, out + 256 * k);
+            continue;
+        }
+        for (int t = 0; t < GEON_CURV_TEAM; ++t) {
+            double r2[2];
+            curv_ricci_lane<Alcubierre>(P, row, t, r2);
+            ric[t] = r2[0];
+            ric[t + GEON_CURV_TEAM] = r2[1];
+        }
+        if (what == 1) {
+            memcpy(out + 16 * k, ric, 16 * sizeof(double));
+            continue;
+        }
+        for (int t = 0; t < GEON_CURV_TEAM; ++t) {
+            double e[2];
+            curv_alc_outputs_lane(P, sf[0], ric, t, what, factor, e);
+            if (what == 4) {
+                if (t == 0) out[k] = e[0];
+                continue;
+            }
+            out[16 * k + t] = e[0];
+            out[16 * k + t + GEON_CURV_TEAM] = e[1];
+        }
+    }
+    return 0;
+}

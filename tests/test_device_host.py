@@ -1,0 +1,120 @@
+"""The kernels' FP64 device code compiled for the host (tests/host/
+device_host_driver.cpp) against the oracle, bit for bit.
+
+The device formulation skips structural zeros, specialises the 4×4 inverse
+to each metric's sparsity and folds zero-Γ steps; none of that may change a
+result. On the host, with glibc's transcendentals on both sides, Minkowski,
+Schwarzschild and Alcubierre traces and Γ therefore equal the oracle's
+exactly. Morris–Thorne takes sin(θ ± h) by angle addition and sin/cos θ from
+its own Cody–Waite routine (both within ulps of libm), so it is held to
+closeness instead. The Alcubierre curvature team (geon_curv.hpp: 31 shared
+metric points, seven Γ on seven lanes, outputs spread over eight lanes),
+run lane after lane, equals the oracle's curvature exactly in all five
+outputs.
+"""
+import ctypes as C
+import math
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+
+
+@pytest.fixture(scope="module")
+def hd(tmp_path_factory):
+    if shutil.which("g++") is None:
+        pytest.skip("no host compiler")
+    so = str(tmp_path_factory.mktemp("hd") / "libhd.so")
+    b = subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off",
+                        "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
+                        f"-I{os.path.join(ROOT, 'include')}",
+                        os.path.join(ROOT, "tests", "host", "device_host_driver.cpp"),
+                        os.path.join(ROOT, "geon_amd", "csrc", "host", "geon_runtime.cpp"),
+                        f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-o", so],
+                       capture_output=True, text=True, timeout=600)
+    assert b.returncode == 0, b.stderr[-3000:]
+    return C.CDLL(so)
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _trace(hd, m, y0, h, max_steps):
+    cls = np.zeros(len(y0), np.int64)
+    steps = np.zeros(len(y0), np.int64)
+    Y = np.array(y0, dtype=np.float64)
+    for i in range(len(Y)):
+        y = np.ascontiguousarray(Y[i])
+        s = C.c_uint32()
+        cls[i] = hd.hd_trace(C.byref(m), _p(y), C.c_double(h), C.c_uint32(max_steps), C.byref(s))
+        steps[i] = s.value
+        Y[i] = y
+    return cls, steps, Y
+
+
+CASES = [
+    ("schwarzschild", dict(horizon_pad=1.005, escape_radius=100.0), [0, 30, math.pi / 2, 0], 4000),
+    ("alcubierre", dict(bubble_thickness=0.125, ship_velocity=0.5, escape_radius=60.0),
+     [0, 10, 0, 0], 3000),
+    ("minkowski", dict(escape_radius=100.0), [0, 20, 0, 0], 3000),
+    ("morris_thorne", dict(escape_radius=60.0), [0, 15, math.pi / 2, 0], 3000),
+]
+
+
+@pytest.mark.parametrize("name,kw,obs,max_steps", CASES, ids=[c[0] for c in CASES])
+def test_traces_match_oracle(hd, O, name, kw, obs, max_steps):
+    kind = {"minkowski": O.MINKOWSKI, "schwarzschild": O.SCHWARZSCHILD,
+            "alcubierre": O.ALCUBIERRE, "morris_thorne": O.MORRIS_THORNE}[name]
+    m = O.metric(kind, **kw)
+    y0 = O.camera_rays(m, O.camera(obs, 60.0, 16, 16))
+    ref = O.trace_rays(m, y0, 0.05, max_steps)
+    cls, steps, Y = _trace(hd, m, y0, 0.05, max_steps)
+    assert np.array_equal(cls, ref["cls"]) and np.array_equal(steps, ref["steps"])
+    if kind == O.MORRIS_THORNE:
+        assert np.abs(Y - ref["y"]).max() < 1e-8
+    else:
+        assert np.array_equal(Y, ref["y"])
+
+
+def test_christoffel_matches_oracle(hd, O):
+    rng = np.random.default_rng(3)
+    for kind, m, lo, hi in [
+        (O.SCHWARZSCHILD, O.metric(O.SCHWARZSCHILD), [2.5, 0.2, -3.0], [30.0, 2.9, 3.0]),
+        (O.ALCUBIERRE, O.metric(O.ALCUBIERRE), [-4.0, -4.0, -4.0], [4.0, 4.0, 4.0]),
+        (O.ALCUBIERRE, O.metric(O.ALCUBIERRE, bubble_thickness=0.125, ship_velocity=0.5),
+         [-4.0, -4.0, -4.0], [4.0, 4.0, 4.0]),
+        (O.MORRIS_THORNE, O.metric(O.MORRIS_THORNE), [-4.0, 0.2, -3.0], [4.0, 2.9, 3.0]),
+    ]:
+        for _ in range(50):
+            x = np.r_[0.0, rng.uniform(lo, hi)]
+            got = np.zeros(64)
+            assert hd.hd_christoffel(C.byref(m), _p(x), _p(got)) == 0
+            ref = np.asarray(O.christoffel(m, x)).reshape(64)
+            if kind == O.MORRIS_THORNE:
+                assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
+            else:
+                assert np.array_equal(got, ref), (kind, x)
+
+
+def test_alcubierre_curvature_team_matches_oracle(hd, O):
+    rng = np.random.default_rng(1)
+    factor = 1.0 / (8.0 * math.pi)
+    for m in (O.metric(O.ALCUBIERRE, bubble_thickness=0.125, ship_velocity=0.5),
+              O.metric(O.ALCUBIERRE)):
+        # random points, plus the wall, the centre and the flat region
+        x = np.r_[np.c_[np.zeros(200), rng.uniform(-4, 4, (200, 3))],
+                  [[0, 2.0, 0, 0], [0, 0, 0, 0], [0, -1.9, 0.3, 0.1], [0, 6.0, 1.0, -2.0]]]
+        x = np.ascontiguousarray(x)
+        for what, width in ((O.CURV_STRESS, 16), (O.CURV_RIEMANN, 256), (O.CURV_SCALAR, 1),
+                            (O.CURV_RICCI, 16), (O.CURV_EINSTEIN, 16)):
+            got = np.zeros(len(x) * width)
+            assert hd.hd_curvature_alc_team(C.byref(m), _p(x), C.c_long(len(x)), what,
+                                            C.c_double(factor), _p(got)) == 0
+            ref, _ = O.curvature(m, x, what)
+            assert np.array_equal(got, np.asarray(ref).reshape(-1)), what

@@ -44,3 +44,23 @@ def test_counts_scale_with_metric_work(counts):
     assert rs["morris_thorne"]["sin"] == 4 and rs["morris_thorne"]["cos"] == 4
     # beyond the Alcubierre wall Γ is exactly zero: the folded step plus the test
     assert rs["alcubierre_flat"]["ops"] == 5 + 4 * 9 + 19
+
+
+def test_curvature_counts(counts):
+    """Per-point T_μν counts: Minkowski has no nonzero Γ, so no work at all;
+    T = G/(8π) adds one product per computed entry (all 16 for Alcubierre);
+    the transcendentals pin the evaluation structure — Γ at the point and at
+    x ± h along each coordinate Γ depends on (Schwarzschild and Morris–Thorne:
+    r/l and θ, 5 evaluations with one sincos each; Alcubierre: x, y, z, 7
+    evaluations of FD Γ, each over 7 metric evaluations with one square root
+    and two tanh) plus the metric twice for g⁻¹ and G."""
+    st, ei = counts["curvature_stress"], counts["curvature_einstein"]
+    assert st["minkowski"]["ops"] == 0 and ei["minkowski"]["ops"] == 0
+    for k in ("schwarzschild", "morris_thorne", "alcubierre"):
+        d = st[k]["ops"] - ei[k]["ops"]
+        assert d == st[k]["mul"] - ei[k]["mul"] and 0 < d <= 16
+    assert st["alcubierre"]["ops"] - ei["alcubierre"]["ops"] == 16
+    for k in ("schwarzschild", "morris_thorne"):
+        assert st[k]["sin"] == 5 + 2 and st[k]["cos"] == 5
+    assert st["alcubierre"]["tanh"] == 2 * (7 * 7 + 2)
+    assert st["alcubierre"]["sqrt"] == 7 * 7 + 2
