@@ -75,7 +75,7 @@ GEON_D double sconst(double c) {
 // cosine (the reduced argument's own rounding and the final sums; checked
 // against 40-digit arithmetic by the test suite) at about half the
 // instructions of the general library routine.
-#ifndef GEON_OPCOUNT   // the op-counting build tallies this as one sine and one cosine
+#ifndef GEON_OPCOUNT   // the op-counting build tallies these as one sine and cosine, one tanh
 GEON_D void sincos_cw(double x, double* s, double* c) {
     if (!(fabs(x) < 0x1p20)) {
         sincos(x, s, c);
@@ -108,6 +108,63 @@ GEON_D void sincos_cw(double x, double* s, double* c) {
     const double cv = (q & 1) ? sr : cr;
     *s = (q & 2) ? -sv : sv;
     *c = ((q + 1) & 2) ? -cv : cv;
+}
+
+// tanh x in about a quarter of the instructions of the general library
+// routine, within about an ulp (checked against 40-digit arithmetic by the
+// test suite). With a = min(|x|, 22) (tanh rounds to 1 from 19.06 on):
+// e^{2a} = 2^k·(1 + p) by the Cody–Waite reduction 2a = k·ln 2 + r (ln 2 in
+// two parts; the first product is exact inside an FMA, and r = 2a when
+// k = 0) and p = e^r − 1 from its Taylor polynomial to r^13 (truncation
+// below 2^-56 relative on |r| ≤ ln2/2). The denominator D = 2^k(1 + p) + 1
+// is kept as an exact two-word sum, q = 2/D comes from a Newton-refined
+// reciprocal and the exact FMA remainder, and tanh a = 1 − q is summed with
+// its rounding error carried, so the final rounding dominates the error.
+// The rounding errors of r and of p = r + r²·P(r) enter the low word of D.
+// NaN stays NaN; the sign of x is copied (tanh(−0) = −0).
+GEON_D double tanh_cw(double x) {
+    const double ax = fabs(x);
+    const double a = ax > 22.0 ? 22.0 : ax;
+    const double t = a + a;
+    const double k = rint(t * 0x1.71547652b82fep+0);            // t / ln 2
+    const double r1 = __builtin_fma(-k, 0x1.62e42fefa39efp-1, t);   // − k·hi(ln 2), exact
+    const double kl = k * 0x1.abc9e3b39803fp-56;                     // k·lo(ln 2)
+    const double r = r1 - kl;
+    const double rl = (r1 - r) - kl;        // the rounding error of r
+    double P = 0x1.6124613a86d09p-33;                            //  1/13!
+    P = __builtin_fma(P, r, sconst(0x1.1eed8eff8d898p-29));      //  1/12!
+    P = __builtin_fma(P, r, sconst(0x1.ae64567f544e4p-26));      //  1/11!
+    P = __builtin_fma(P, r, sconst(0x1.27e4fb7789f5cp-22));      //  1/10!
+    P = __builtin_fma(P, r, sconst(0x1.71de3a556c734p-19));      //  1/9!
+    P = __builtin_fma(P, r, sconst(0x1.a01a01a01a01ap-16));      //  1/8!
+    P = __builtin_fma(P, r, sconst(0x1.a01a01a01a01ap-13));      //  1/7!
+    P = __builtin_fma(P, r, sconst(0x1.6c16c16c16c17p-10));      //  1/6!
+    P = __builtin_fma(P, r, sconst(0x1.1111111111111p-7));       //  1/5!
+    P = __builtin_fma(P, r, sconst(0x1.5555555555555p-5));       //  1/4!
+    P = __builtin_fma(P, r, sconst(0x1.5555555555555p-3));       //  1/3!
+    P = __builtin_fma(P, r, 0.5);                                //  1/2!
+    const double w = (r * r) * P;
+    const double p = r + w;                 // e^r − 1 = p + pl (+ rl·e^r)
+    const double pl = (r - p) + w;
+    const double s = ldexp(1.0, (int)k);                         // 2^k, 0 ≤ k ≤ 63
+    const double B = s * p;
+    const double u = s + B;                 // s + B exactly as u + ul (|s| ≥ |B|)
+    const double ul = (s - u) + B;
+    const double D = u + 1.0;               // u + 1 exactly as D + Dl (u ≥ 1)
+    const double Dl = (ul + ((u - D) + 1.0)) + s * __builtin_fma(rl, 1.0 + p, pl);
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(D);     // 1/D, refined twice
+#else
+    double y = 1.0 / D;                     // host build (tests)
+#endif
+    y = __builtin_fma(y, __builtin_fma(-D, y, 1.0), y);
+    y = __builtin_fma(y, __builtin_fma(-D, y, 1.0), y);
+    const double q = 2.0 * y;
+    const double ql = __builtin_fma(-q, Dl, __builtin_fma(-q, D, 2.0)) * y;   // 2/(D + Dl) − q
+    const double h = 1.0 - q;               // 1 − q exactly as h + (1 − h) − q (q ≤ 1)
+    const double res = h + (((1.0 - h) - q) - ql);
+    // below 2^-27, x³/3 is under half an ulp of x: tanh x rounds to x
+    return ax < 0x1p-27 ? x : __builtin_copysign(res, x);
 }
 #endif
 
@@ -462,6 +519,18 @@ struct MorrisThorne {
 };
 
 // ---- Alcubierre (alcubierre.rs), generic FD Γ -----------------------------
+// tanh of the shape function: tanh_cw on the GPU. The host build of this
+// header (the parity tests compare it with the oracle bit for bit) takes the
+// C library's, which the oracle uses too, so that it checks the formulation
+// itself; tanh_cw has its own accuracy test.
+GEON_D double shape_tanh(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return tanh_cw(x);
+#else
+    return tanh(x);
+#endif
+}
+
 struct Alcubierre {
     static constexpr int flat_test = 2;   // flat steps decided per step by flat()
     static constexpr bool g_nz(int m, int n) {
@@ -481,11 +550,11 @@ struct Alcubierre {
     GEON_D static double shape(const geon_dmetric& P, double r_s) {
         double tp, tm;
         if (P.s_pow2) {
-            tp = tanh((r_s + P.R) * P.inv_s);
-            tm = tanh((r_s - P.R) * P.inv_s);
+            tp = shape_tanh((r_s + P.R) * P.inv_s);
+            tm = shape_tanh((r_s - P.R) * P.inv_s);
         } else {
-            tp = tanh((r_s + P.R) / P.s);
-            tm = tanh((r_s - P.R) / P.s);
+            tp = shape_tanh((r_s + P.R) / P.s);
+            tm = shape_tanh((r_s - P.R) / P.s);
         }
         return (tp - tm) / P.denom;
     }
@@ -537,17 +606,23 @@ struct Alcubierre {
         if (!__builtin_isfinite(r2)) return DIVERGED;
         return (r2 > P.esc2) ? ESCAPED : RUNNING;
     }
-    // True when all four RK4 stages of the step from y see the exact flat
-    // region (accel's early-out). While Γ = 0 the stage momenta equal p, so
-    // every stage position lies within |h|·|p_spatial| of y and its r_s is at
-    // least r_s(y) − |h|·|p|; the 1e-9 margin covers the rounding of the
-    // stage positions and of r_s. NaN or infinite states fail the test.
-    GEON_D static bool flat(const geon_dmetric& P, const double* y, double h) {
+    // The number of consecutive steps from y, this one included, whose four
+    // RK4 stages all see the exact flat region of accel's early-out (0: not
+    // this step). While Γ = 0 the stage momenta equal p, which the folded
+    // step leaves unchanged, and every step moves the position by h·p up to
+    // rounding; so every stage of K steps lies within K·|h|·|p| of y and its
+    // r_s is at least r_s(y) − K·|h|·|p|. The 1e-9·(1 + r) margin on r_s and
+    // the relative 1e-6 on the distance cover the rounding of the stage
+    // positions and of r_s over K ≤ 65536 steps. NaN or infinite states
+    // give 0.
+    GEON_D static uint32_t flat_steps(const geon_dmetric& P, const double* y, double h) {
         const double r = rs(P, y);
         const double d = r - P.flat_rs - 1e-9 * (1.0 + r);
-        if (!(d > 0.0)) return false;
+        if (!(d > 0.0)) return 0;
         const double p2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
-        return d * d > h * h * p2;
+        if (!(d * d > h * h * p2)) return 0;   // one step
+        const double k = d / (fabs(h) * sqrt(p2) * (1.0 + 1e-6));
+        return k >= 65536.0 ? 65536u : (k >= 1.0 ? (uint32_t)k : 1u);
     }
 };
 
@@ -574,13 +649,26 @@ GEON_D void rk4_step_flat(const double* y, double h, double* out) {
 
 // k_i = (p_i, dp_i); the combination ((k1 + 2k2) + 2k3) + k4 is accumulated
 // in that order. Metrics with flat regions take the folded step there.
+// Steps of a ray still known to be flat from its last flatness test
+// (metrics whose flat regions are tested per step); 0: test again.
+struct FlatRun {
+    uint32_t left = 0;
+};
+
 template <class MP>
-GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out) {
+GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out, FlatRun& fr) {
     if constexpr (MP::flat_test == 1) {
         rk4_step_flat(y, h, out);
         return;
     } else if constexpr (MP::flat_test == 2) {
-        if (MP::flat(P, y, h)) {
+        if (fr.left > 0) {
+            --fr.left;
+            rk4_step_flat(y, h, out);
+            return;
+        }
+        const uint32_t k = MP::flat_steps(P, y, h);
+        if (k > 0) {
+            fr.left = k - 1;
             rk4_step_flat(y, h, out);
             return;
         }
@@ -622,6 +710,12 @@ GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* o
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
+}
+
+template <class MP>
+GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out) {
+    FlatRun fr;
+    rk4_step<MP>(P, y, h, out, fr);
 }
 
 // |g(p,p)| at the final position (geodesic.rs:75-77).
@@ -667,6 +761,7 @@ GEON_D void direction(const double* y, double* d) {
 // pre-step state, max_steps → MAXSTEPS.
 template <class MP>
 GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps, uint32_t* steps) {
+    FlatRun fr;
     for (uint32_t n = 0; n < max_steps; ++n) {
         const int t = MP::terminate(P, y);
         if (t != RUNNING) {
@@ -674,7 +769,7 @@ GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps,
             return t;
         }
         double yn[8];
-        rk4_step<MP>(P, y, h, yn);
+        rk4_step<MP>(P, y, h, yn, fr);
         if (!finite8(yn)) {
             *steps = n;
             return DIVERGED;

@@ -133,8 +133,9 @@ struct Prec<MP, double> {
     using Params = geon_dmetric;
     GEON_D static Params params(const geon_dmetric& m) { return m; }
     GEON_D static int terminate(const Params& P, const double* y) { return MP::terminate(P, y); }
-    GEON_D static float step(const Params& P, const double* y, double h, double* out, float&, float&) {
-        rk4_step<MP>(P, y, h, out);
+    GEON_D static float step(const Params& P, const double* y, double h, double* out, float&, float&,
+                             FlatRun& fr) {
+        rk4_step<MP>(P, y, h, out, fr);
         return 1.0f;
     }
     GEON_D static void trig(const double*, float&, float&) {}
@@ -149,7 +150,8 @@ struct Prec<MP, float> {
     using Params = FParams;
     GEON_D static Params params(const geon_dmetric& m) { return fparams(m); }
     GEON_D static int terminate(const Params& P, const float* y) { return MF::terminate(P, y); }
-    GEON_D static float step(const Params& P, const float* y, float h, float* out, float& s, float& c) {
+    GEON_D static float step(const Params& P, const float* y, float h, float* out, float& s, float& c,
+                             FlatRun&) {
         return rk4_step_f<MF>(P, y, h, out, s, c);
     }
     GEON_D static void trig(const float* y, float& s, float& c) {
@@ -178,6 +180,7 @@ __device__ void sched_run(const geon_sched_args& A) {
     int early[RPL];
     float min_s[RPL], min_r[RPL];   // FP32 conditioning trackers
     float sn[RPL], cs[RPL];         // sin/cos of the current θ (FP32 charts with θ)
+    FlatRun fr[RPL];                // FP64 steps still known to be flat
     Params P[RPL];
     unsigned long long executed = 0;
 
@@ -193,6 +196,7 @@ __device__ void sched_run(const geon_sched_args& A) {
         P[r] = PR::params(J.F.m);
         min_s[r] = 1.0f;
         min_r[r] = 1e30f;
+        fr[r].left = 0;
         act[r] = true;
         if (!resume) {
             act[r] = init_ray<MP, T>(J, gid, A, y[r]);
@@ -308,7 +312,7 @@ __device__ void sched_run(const geon_sched_args& A) {
                 }
                 T yn[8];
                 float s2 = sn[r], c2 = cs[r];
-                const float asn = PR::step(P[r], y[r], h[r], yn, s2, c2);
+                const float asn = PR::step(P[r], y[r], h[r], yn, s2, c2, fr[r]);
                 if (!PR::finite(yn)) {
                     finish(r, DIVERGED);
                     continue;
