@@ -121,8 +121,10 @@ GEON_D void sincos_cw(double x, double* s, double* c) {
 // reciprocal and the exact FMA remainder, and tanh a = 1 − q is summed with
 // its rounding error carried, so the final rounding dominates the error.
 // The rounding errors of r and of p = r + r²·P(r) enter the low word of D.
-// NaN stays NaN; the sign of x is copied (tanh(−0) = −0).
-GEON_D double tanh_cw(double x) {
+// NaN stays NaN; the sign of x is copied (tanh(−0) = −0). Not inlined: the
+// fourteen calls of an Alcubierre Γ, inlined and interleaved, would push the
+// ray kernels past 256 VGPRs.
+__device__ __attribute__((noinline)) double tanh_cw(double x) {
     const double ax = fabs(x);
     const double a = ax > 22.0 ? 22.0 : ax;
     const double t = a + a;

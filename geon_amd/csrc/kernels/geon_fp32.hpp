@@ -81,7 +81,6 @@ GEON_D float ftanh(float u) {
 // ---- Minkowski --------------------------------------------------------------
 struct MinkowskiF {
     static constexpr int flat_test = 1;
-    GEON_D static bool flat(const FParams&, const float*, float) { return true; }
     GEON_D static void accel(const FParams&, const float*, float* dp, float, float) {
         dp[0] = dp[1] = dp[2] = dp[3] = 0.0f;
     }
@@ -193,13 +192,21 @@ struct MorrisThorneF {
 struct AlcubierreF {
     static constexpr bool uses_theta = false;
     static constexpr int flat_test = 2;
-    // Every stage of the step from y stays beyond flat_rs, where Γ is zero.
-    GEON_D static bool flat(const FParams& P, const float* y, float h) {
+    // The number of consecutive steps from y, this one included, whose
+    // stages all stay beyond flat_rs, where Γ is zero (0: not this one).
+    // The momentum is constant there and each step moves the position by
+    // h·p, so K steps stay within K·|h|·|p| of y. The allowance 1e-3·(1 + r)
+    // on the distance covers the FP32 rounding of K ≤ 1024 positions
+    // (≤ 1024·2^-24·r).
+    GEON_D static uint32_t flat_steps(const FParams& P, const float* y, float h) {
 #pragma clang fp contract(fast)
         const float dx = y[1] - P.ship_x, dy = y[2], dz = y[3];
-        const float d = fsqrt(dx * dx + dy * dy + dz * dz) - P.flat_rs;
+        const float r = fsqrt(dx * dx + dy * dy + dz * dz);
+        const float d = r - P.flat_rs;
         const float p2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
-        return d > 0.0f && d * d > h * h * p2;
+        if (!(d > 0.0f && d * d > h * h * p2)) return 0;   // one step
+        const float k = (d - 1e-3f * (1.0f + r)) * frcp(fabsf(h) * fsqrt(p2) * 1.001f);
+        return k >= 1024.0f ? 1024u : (k >= 1.0f ? (uint32_t)k : 1u);
     }
     GEON_D static void shape(const FParams& P, const float* y, float& f, float& fp, float* n) {
 #pragma clang fp contract(fast)
@@ -279,12 +286,24 @@ GEON_D void rotate_sc(float s, float c, float theta_new, float d, float& so, flo
 // at the end of the step. Returns |sin θ| of the new state (1 for Cartesian
 // charts).
 template <class MF>
-GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c) {
+GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c,
+                        FlatRun& fr) {
 #pragma clang fp contract(fast)
     if constexpr (MF::flat_test != 0) {
         // Γ = 0 at every stage: the RK4 step is the straight line y + h·p
         // (momenta, p^t included, unchanged)
-        if (MF::flat_test == 1 || MF::flat(P, y, h)) {
+        bool flat = MF::flat_test == 1;
+        if constexpr (MF::flat_test == 2) {
+            if (fr.left > 0) {
+                --fr.left;
+                flat = true;
+            } else {
+                const uint32_t k = MF::flat_steps(P, y, h);
+                fr.left = k > 0 ? k - 1 : 0;
+                flat = k > 0;
+            }
+        }
+        if (flat) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 out[i] = y[i] + h * y[4 + i];
@@ -342,6 +361,12 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
         MF::reproject(P, out);
         return 1.0f;
     }
+}
+
+template <class MF>
+GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c) {
+    FlatRun fr;
+    return rk4_step_f<MF>(P, y, h, out, s, c, fr);
 }
 
 GEON_D bool finite8f(const float* y) {

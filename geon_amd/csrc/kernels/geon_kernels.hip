@@ -76,22 +76,25 @@ __device__ void trace_states(uint32_t slot, uint64_t n, const double* __restrict
     }
 }
 
-#define GEON_KERNELS(NAME, MP)                                                                  \
-    extern "C" __global__ __launch_bounds__(256) void geon_render_rays_##NAME(                  \
+// WAVES: the minimum waves per SIMD the register allocation must allow.
+// Alcubierre's full step (seven metric evaluations per stage) otherwise takes
+// over 256 VGPRs, one wave per SIMD; at four it spills four.
+#define GEON_KERNELS(NAME, MP, WAVES)                                                           \
+    extern "C" __global__ __launch_bounds__(256, WAVES) void geon_render_rays_##NAME(           \
         uint32_t slot, uint64_t n_rays, int32_t* cls, uint32_t* steps, double* resid,          \
         double* dir) {                                                                          \
         render_rays<MP>(slot, n_rays, cls, steps, resid, dir);                                  \
     }                                                                                           \
-    extern "C" __global__ __launch_bounds__(256) void geon_trace_states_##NAME(                 \
+    extern "C" __global__ __launch_bounds__(256, WAVES) void geon_trace_states_##NAME(          \
         uint32_t slot, uint64_t n, const double* y0, int32_t* cls, uint32_t* steps,            \
         double* yf, double* resid, double* dir) {                                               \
         trace_states<MP>(slot, n, y0, cls, steps, yf, resid, dir);                              \
     }
 
-GEON_KERNELS(mink, Minkowski)
-GEON_KERNELS(schw, Schwarzschild)
-GEON_KERNELS(alc, Alcubierre)
-GEON_KERNELS(mt, MorrisThorne)
+GEON_KERNELS(mink, Minkowski, 1)
+GEON_KERNELS(schw, Schwarzschild, 1)
+GEON_KERNELS(alc, Alcubierre, 4)
+GEON_KERNELS(mt, MorrisThorne, 1)
 
 // ---- per-pixel reduction, sky colour and frame statistics ------------------
 // render.rs:96-163: colour per sub-ray, f32 average, majority class with

@@ -151,8 +151,8 @@ struct Prec<MP, float> {
     GEON_D static Params params(const geon_dmetric& m) { return fparams(m); }
     GEON_D static int terminate(const Params& P, const float* y) { return MF::terminate(P, y); }
     GEON_D static float step(const Params& P, const float* y, float h, float* out, float& s, float& c,
-                             FlatRun&) {
-        return rk4_step_f<MF>(P, y, h, out, s, c);
+                             FlatRun& fr) {
+        return rk4_step_f<MF>(P, y, h, out, s, c, fr);
     }
     GEON_D static void trig(const float* y, float& s, float& c) {
         if constexpr (MF::uses_theta) sincosf(y[2], &s, &c);
@@ -180,7 +180,7 @@ __device__ void sched_run(const geon_sched_args& A) {
     int early[RPL];
     float min_s[RPL], min_r[RPL];   // FP32 conditioning trackers
     float sn[RPL], cs[RPL];         // sin/cos of the current θ (FP32 charts with θ)
-    FlatRun fr[RPL];                // FP64 steps still known to be flat
+    FlatRun fr[RPL];                // steps still known to be flat
     Params P[RPL];
     unsigned long long executed = 0;
 

@@ -105,6 +105,7 @@ struct cnt_f {
     cnt_f(float x) : v(x) {}
     explicit cnt_f(cnt_d x) : v((float)x.v) {}
     explicit operator float() const { return v; }
+    explicit operator uint32_t() const { return (uint32_t)v; }
     cnt_f& operator+=(cnt_f o) { ++T.add; v += o.v; return *this; }
     cnt_f& operator-=(cnt_f o) { ++T.sub; v -= o.v; return *this; }
     cnt_f& operator*=(cnt_f o) { ++T.mul; v *= o.v; return *this; }
@@ -188,13 +189,15 @@ static OpTally step_ops(const geon_dmetric& P, const double* y0, bool flat_known
 }
 
 template <class MF>
-static OpTally step_ops_f(const geon_dmetric& Pd, const double* y0) {
+static OpTally step_ops_f(const geon_dmetric& Pd, const double* y0, bool flat_known = false) {
     const FParams P = fparams(Pd);
     cnt_f y[8], out[8];
     for (int i = 0; i < 8; ++i) y[i] = cnt_f((float)y0[i]);
     cnt_f s = cnt_f(::sinf((float)y0[2])), c = cnt_f(::cosf((float)y0[2]));
+    FlatRun fr;
+    fr.left = flat_known ? 1 : 0;
     memset(&T, 0, sizeof T);
-    rk4_step_f<MF>(P, y, cnt_f(0.05f), out, s, c);
+    rk4_step_f<MF>(P, y, cnt_f(0.05f), out, s, c, fr);
     return T;
 }
 
@@ -296,7 +299,7 @@ int main() {
     emit("schwarzschild", step_ops_f<SchwarzschildF>(metric(1), ys), false);
     emit("morris_thorne", step_ops_f<MorrisThorneF>(metric(3), ym), false);
     emit("alcubierre_wall", step_ops_f<AlcubierreF>(metric(2), ya_wall), false);
-    emit("alcubierre_flat", step_ops_f<AlcubierreF>(metric(2), ya_flat), true);
+    emit("alcubierre_flat", step_ops_f<AlcubierreF>(metric(2), ya_flat, true), true);
     // curvature sample points: no branch of any Γ depends on the point here
     const double xs[4] = {0.0, 10.0, 1.2, 0.3};
     const double xa[4] = {0.0, 1.9, 0.4, -0.3};
