@@ -95,7 +95,7 @@ struct geon_sched_args {
     void* slot_y;                         // SoA 8 × n_slots of T (quantum mode)
     unsigned long long* slot_g;           // ray id per slot, ~0 = empty
     uint32_t* slot_n;                     // steps taken per slot
-    float* slot_aux;                      // SoA 2 × n_slots: FP32 conditioning trackers
+    float* slot_aux;                      // SoA 8 × n_slots: FP32 conditioning trackers, flat run
     unsigned long long* fb_list;          // FP64 fallback ray ids (FP32 kernels), a ring
     unsigned long long* fb_count;         // entries ever pushed
     unsigned long long fb_cap;            // ring capacity

@@ -215,7 +215,7 @@ int init_queue(geon_server* s, Queue& Q, int kind, int prec, uint32_t cap_jobs, 
     GEON_HIP(hipMalloc(&Q.slot_y, (size_t)Q.n_slots * 8 * tsz), "slots");
     GEON_HIP(hipMalloc((void**)&Q.slot_g, (size_t)Q.n_slots * 8), "slots");
     GEON_HIP(hipMalloc((void**)&Q.slot_n, (size_t)Q.n_slots * 4), "slots");
-    GEON_HIP(hipMalloc((void**)&Q.slot_aux, (size_t)Q.n_slots * 8), "slots");
+    GEON_HIP(hipMalloc((void**)&Q.slot_aux, (size_t)Q.n_slots * 8 * 4), "slots");
     uint64_t n = Q.n_slots;
     unsigned long long empty = ~0ull;
     void* a[] = {&Q.slot_g, &n, &empty};

@@ -285,32 +285,52 @@ GEON_D void rotate_sc(float s, float c, float theta_new, float d, float& so, flo
 // come from rotations by the small stage increments δ. p^t is re-projected
 // at the end of the step. Returns |sin θ| of the new state (1 for Cartesian
 // charts).
+// A run of straight-line (Γ = 0) FP32 steps: the steps still known to be
+// flat, the steps taken since the run began and the position it began at.
+// Step j of a run puts the ray at a + (j·h)·p, one rounding from the exact
+// line: adding h·p step by step would round the position every step, and
+// far from the origin (an FP32 ulp is 4e-6 at |x| = 40) that drift alone
+// moved escape directions by up to 2e-3 rad after a few hundred steps.
+struct FlatRunF {
+    uint32_t left = 0;
+    uint32_t j = 0;
+    float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+};
+
 template <class MF>
 GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c,
-                        FlatRun& fr) {
+                        FlatRunF& fr) {
 #pragma clang fp contract(fast)
     if constexpr (MF::flat_test != 0) {
         // Γ = 0 at every stage: the RK4 step is the straight line y + h·p
         // (momenta, p^t included, unchanged)
-        bool flat = MF::flat_test == 1;
+        bool cont = fr.j > 0, flat = true;
         if constexpr (MF::flat_test == 2) {
             if (fr.left > 0) {
                 --fr.left;
-                flat = true;
             } else {
                 const uint32_t k = MF::flat_steps(P, y, h);
                 fr.left = k > 0 ? k - 1 : 0;
                 flat = k > 0;
+                cont = false;
             }
         }
         if (flat) {
+            if (!cont) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) fr.a[i] = y[i];
+                fr.j = 0;
+            }
+            ++fr.j;
+            const float jh = (float)fr.j * h;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                out[i] = y[i] + h * y[4 + i];
+                out[i] = fr.a[i] + jh * y[4 + i];
                 out[4 + i] = y[4 + i];
             }
             return 1.0f;
         }
+        fr.j = 0;
     }
     float acc[8], yt[8], dp[4];
     float ss = s, cs = c;
@@ -365,7 +385,7 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
 
 template <class MF>
 GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c) {
-    FlatRun fr;
+    FlatRunF fr;
     return rk4_step_f<MF>(P, y, h, out, s, c, fr);
 }
 

@@ -131,6 +131,7 @@ struct Prec;
 template <class MP>
 struct Prec<MP, double> {
     using Params = geon_dmetric;
+    using Run = FlatRun;
     GEON_D static Params params(const geon_dmetric& m) { return m; }
     GEON_D static int terminate(const Params& P, const double* y) { return MP::terminate(P, y); }
     GEON_D static float step(const Params& P, const double* y, double h, double* out, float&, float&,
@@ -139,6 +140,13 @@ struct Prec<MP, double> {
         return 1.0f;
     }
     GEON_D static void trig(const double*, float&, float&) {}
+    // a parked ray keeps its flat run (plane 2 of the slot's auxiliary words)
+    GEON_D static void park_run(const Run& fr, float* aux, size_t n, uint32_t s) {
+        if constexpr (MP::flat_test == 2) aux[2 * n + s] = __uint_as_float(fr.left);
+    }
+    GEON_D static void resume_run(Run& fr, const float* aux, size_t n, uint32_t s) {
+        if constexpr (MP::flat_test == 2) fr.left = __float_as_uint(aux[2 * n + s]);
+    }
     GEON_D static float radius(const double*) { return 1e30f; }
     GEON_D static bool finite(const double* y) { return finite8(y); }
     GEON_D static bool suspicious(const Params&, int, const double*, float, float) { return false; }
@@ -148,14 +156,33 @@ template <class MP>
 struct Prec<MP, float> {
     using MF = typename FP32Of<MP>::type;
     using Params = FParams;
+    using Run = FlatRunF;
     GEON_D static Params params(const geon_dmetric& m) { return fparams(m); }
     GEON_D static int terminate(const Params& P, const float* y) { return MF::terminate(P, y); }
     GEON_D static float step(const Params& P, const float* y, float h, float* out, float& s, float& c,
-                             FlatRun& fr) {
+                             FlatRunF& fr) {
         return rk4_step_f<MF>(P, y, h, out, s, c, fr);
     }
     GEON_D static void trig(const float* y, float& s, float& c) {
         if constexpr (MF::uses_theta) sincosf(y[2], &s, &c);
+    }
+    // a parked ray keeps its flat run, anchor included (planes 2–7), so the
+    // positions it takes after resuming do not depend on where it was parked
+    GEON_D static void park_run(const Run& fr, float* aux, size_t n, uint32_t s) {
+        if constexpr (MF::flat_test != 0) {
+            aux[2 * n + s] = __uint_as_float(fr.left);
+            aux[3 * n + s] = __uint_as_float(fr.j);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) aux[(4 + i) * n + s] = fr.a[i];
+        }
+    }
+    GEON_D static void resume_run(Run& fr, const float* aux, size_t n, uint32_t s) {
+        if constexpr (MF::flat_test != 0) {
+            fr.left = __float_as_uint(aux[2 * n + s]);
+            fr.j = __float_as_uint(aux[3 * n + s]);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) fr.a[i] = aux[(4 + i) * n + s];
+        }
     }
     GEON_D static float radius(const float* y) { return MF::radius(y); }
     GEON_D static bool finite(const float* y) { return finite8f(y); }
@@ -180,7 +207,7 @@ __device__ void sched_run(const geon_sched_args& A) {
     int early[RPL];
     float min_s[RPL], min_r[RPL];   // FP32 conditioning trackers
     float sn[RPL], cs[RPL];         // sin/cos of the current θ (FP32 charts with θ)
-    FlatRun fr[RPL];                // steps still known to be flat
+    typename PR::Run fr[RPL];       // flat-step runs
     Params P[RPL];
     unsigned long long executed = 0;
 
@@ -196,7 +223,7 @@ __device__ void sched_run(const geon_sched_args& A) {
         P[r] = PR::params(J.F.m);
         min_s[r] = 1.0f;
         min_r[r] = 1e30f;
-        fr[r].left = 0;
+        fr[r] = typename PR::Run{};
         act[r] = true;
         if (!resume) {
             act[r] = init_ray<MP, T>(J, gid, A, y[r]);
@@ -233,6 +260,7 @@ __device__ void sched_run(const geon_sched_args& A) {
                 if (A.slot_aux) {
                     min_s[r] = A.slot_aux[s];
                     min_r[r] = A.slot_aux[A.n_slots + s];
+                    PR::resume_run(fr[r], A.slot_aux, A.n_slots, s);
                 }
             }
         }
@@ -349,6 +377,7 @@ __device__ void sched_run(const geon_sched_args& A) {
                 if (A.slot_aux) {
                     A.slot_aux[s] = min_s[r];
                     A.slot_aux[A.n_slots + s] = min_r[r];
+                    PR::park_run(fr[r], A.slot_aux, A.n_slots, s);
                 }
             } else {
                 A.slot_g[s] = ~0ull;

@@ -194,8 +194,9 @@ static OpTally step_ops_f(const geon_dmetric& Pd, const double* y0, bool flat_kn
     cnt_f y[8], out[8];
     for (int i = 0; i < 8; ++i) y[i] = cnt_f((float)y0[i]);
     cnt_f s = cnt_f(::sinf((float)y0[2])), c = cnt_f(::cosf((float)y0[2]));
-    FlatRun fr;
+    FlatRunF fr;
     fr.left = flat_known ? 1 : 0;
+    fr.j = flat_known ? 1 : 0;
     memset(&T, 0, sizeof T);
     rk4_step_f<MF>(P, y, cnt_f(0.05f), out, s, c, fr);
     return T;

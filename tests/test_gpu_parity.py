@@ -116,6 +116,24 @@ def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps, cha
         assert (a <= ANGLE_TOL).mean() >= 0.90 and a.max() <= 1e-6
 
 
+def test_alcubierre_long_flat_runs(ctx, O):
+    """A narrow camera 40 units out on the bubble's axis: every ray first
+    crosses ~35 units of exactly flat space (flatness tested once per run of
+    up to ~700 steps), then most of them are deflected by the wall. Classes
+    and escape directions against the oracle."""
+    import geon_amd as geon
+    om = O.metric(O.ALCUBIERRE, bubble_thickness=0.125, ship_velocity=0.5, escape_radius=100.0)
+    y0 = O.camera_rays(om, O.camera((0.0, 40.0, 0.0, 0.0), 8.0, 48, 48))
+    ref = O.trace_rays(om, y0, 0.05, 4000)
+    got = geon.trace_rays(om, y0, 0.05, 4000, ctx=ctx)
+    bad, ang = _compare_rays(O, om, y0, got, ref, 100.0, "alcubierre far camera")
+    assert bad == 0 and ang <= ANGLE_TOL
+    # the wall deflected most rays (their final direction is not the initial one)
+    p0 = y0[:, 5:8] / np.linalg.norm(y0[:, 5:8], axis=1)[:, None]
+    defl = np.arccos(np.clip((p0 * ref["dir"]).sum(axis=1), -1.0, 1.0))
+    assert (ref["cls"] == O.ESCAPED).all() and (defl > 1e-6).mean() > 0.5
+
+
 def test_trace_semantics_on_gpu(ctx, O):
     """terminate-first, NaN guard with the pre-step state, max_steps."""
     import geon_amd as geon
