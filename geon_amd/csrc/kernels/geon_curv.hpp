@@ -132,7 +132,7 @@ struct AlcStencil {
 GEON_D double curv_alc_point_f(const geon_dmetric& P, const double* x, int j) {
     double xe[4];
     AlcStencil::point(P, x, j, xe);
-    return Alcubierre::shape(P, Alcubierre::rs(P, xe));
+    return Alcubierre::shape<true>(P, Alcubierre::rs(P, xe));
 }
 
 // phase B: Γ at Γ-stencil point t (< 7) from the staged f values, as

@@ -121,10 +121,8 @@ GEON_D void sincos_cw(double x, double* s, double* c) {
 // reciprocal and the exact FMA remainder, and tanh a = 1 − q is summed with
 // its rounding error carried, so the final rounding dominates the error.
 // The rounding errors of r and of p = r + r²·P(r) enter the low word of D.
-// NaN stays NaN; the sign of x is copied (tanh(−0) = −0). Not inlined: the
-// fourteen calls of an Alcubierre Γ, inlined and interleaved, would push the
-// ray kernels past 256 VGPRs.
-__device__ __attribute__((noinline)) double tanh_cw(double x) {
+// NaN stays NaN; the sign of x is copied (tanh(−0) = −0).
+GEON_D double tanh_cw(double x) {
     const double ax = fabs(x);
     const double a = ax > 22.0 ? 22.0 : ax;
     const double t = a + a;
@@ -168,6 +166,9 @@ __device__ __attribute__((noinline)) double tanh_cw(double x) {
     // below 2^-27, x³/3 is under half an ulp of x: tanh x rounds to x
     return ax < 0x1p-27 ? x : __builtin_copysign(res, x);
 }
+// The same as a call: the fourteen tanh of an Alcubierre Γ, inlined and
+// interleaved, would push the ray kernels past 256 VGPRs.
+__device__ __attribute__((noinline)) double tanh_cw_call(double x) { return tanh_cw(x); }
 #endif
 
 GEON_D bool finite8(const double* y) {
@@ -525,9 +526,12 @@ struct MorrisThorne {
 // header (the parity tests compare it with the oracle bit for bit) takes the
 // C library's, which the oracle uses too, so that it checks the formulation
 // itself; tanh_cw has its own accuracy test.
+// INLINE: inline the routine (kernels with registers to spare) or call it.
+template <bool INLINE = false>
 GEON_D double shape_tanh(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    return tanh_cw(x);
+    if constexpr (INLINE) return tanh_cw(x);
+    else return tanh_cw_call(x);
 #else
     return tanh(x);
 #endif
@@ -549,14 +553,15 @@ struct Alcubierre {
         return sqrt(dx * dx + dy * dy + dz * dz);
     }
     // alcubierre.rs:33-44 — denom = 2·tanh(R/s) precomputed on the host
+    template <bool INLINE_TANH = false>
     GEON_D static double shape(const geon_dmetric& P, double r_s) {
         double tp, tm;
         if (P.s_pow2) {
-            tp = shape_tanh((r_s + P.R) * P.inv_s);
-            tm = shape_tanh((r_s - P.R) * P.inv_s);
+            tp = shape_tanh<INLINE_TANH>((r_s + P.R) * P.inv_s);
+            tm = shape_tanh<INLINE_TANH>((r_s - P.R) * P.inv_s);
         } else {
-            tp = shape_tanh((r_s + P.R) / P.s);
-            tm = shape_tanh((r_s - P.R) / P.s);
+            tp = shape_tanh<INLINE_TANH>((r_s + P.R) / P.s);
+            tm = shape_tanh<INLINE_TANH>((r_s - P.R) / P.s);
         }
         return (tp - tm) / P.denom;
     }
