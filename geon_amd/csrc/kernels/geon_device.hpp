@@ -8,9 +8,12 @@
 // skip masked terms at compile time. Adding an exact zero never changes an
 // IEEE sum, so the skipped work leaves the results identical to the dense
 // reference loops (geodesic.rs:49-57, metric/mod.rs:55-66) while costing
-// only the nonzero terms. The FP64 build uses no FMA contraction. One
-// deliberate exception: Morris–Thorne's sin(θ ± h) come from angle addition
-// (see MorrisThorne::christoffel).
+// only the nonzero terms. The FP64 build uses no FMA contraction.
+// Deliberate exceptions, each at the level of a transcendental's last bit:
+// Morris–Thorne's sin(θ ± h) come from angle addition (see
+// MorrisThorne::christoffel), and on the GPU its sin/cos θ and Alcubierre's
+// tanh come from the routines below (sincos_cw, tanh_cw) rather than the
+// library's.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -170,6 +173,18 @@ GEON_D double tanh_cw(double x) {
 // interleaved, would push the ray kernels past 256 VGPRs.
 __device__ __attribute__((noinline)) double tanh_cw_call(double x) { return tanh_cw(x); }
 #endif
+
+// sin θ and cos θ of the spherical charts: sincos_cw on the GPU. The host
+// build of this header (the parity tests compare it with the oracle bit for
+// bit) takes the C library's, which the oracle uses too, so that it checks
+// the formulation itself; sincos_cw has its own accuracy test.
+GEON_D void theta_sincos(double x, double* s, double* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    sincos_cw(x, s, c);
+#else
+    sincos(x, s, c);
+#endif
+}
 
 GEON_D bool finite8(const double* y) {
     bool ok = true;
@@ -384,6 +399,10 @@ struct Schwarzschild {
         // 2M/r from 1/r; a non-finite r or 1/r leaves f non-finite as the
         // IEEE division would, and the step is discarded as diverged either way
         const double f = 1.0 - div_rn(two_m, r, inv_r);
+        // the library sincos: rays that orbit near the photon sphere amplify
+        // the ~1-ulp error of sincos_cw into their escape directions (2.6e-8
+        // rad at r = 12M against the oracle), where the library keeps them
+        // near 1e-13
         double sin_t, cos_t;
         sincos(x[2], &sin_t, &cos_t);
         const double r2 = r * r;
@@ -472,7 +491,7 @@ struct MorrisThorne {
     // The generic Γ of metric/mod.rs:51-69 with ∂g by central differences
     // (metric/mod.rs:25-43) written out for this metric: g_θθ = b0² + l² and
     // g_φφ = g_θθ sin²θ, so the centre and the l ± h evaluations share
-    // sin θ and b0². sin θ and cos θ come from sincos_cw; sin(θ ± h) from
+    // sin θ and b0². sin θ and cos θ come from theta_sincos; sin(θ ± h) from
     // rotating them by the constants sin h, cos h (angle addition) instead
     // of two more sines:
     // each differs from a directly evaluated sine by at most an ulp, the
@@ -483,7 +502,7 @@ struct MorrisThorne {
         const double l = x[1], th = x[2], h = P.fd_h, two_h = P.fd_two_h;
         const double b2 = P.b0 * P.b0;
         double st, ct;
-        sincos_cw(th, &st, &ct);
+        theta_sincos(th, &st, &ct);
         const double r2 = b2 + l * l;
         double g0[16], gi[16], dgv[64];
         g0[0] = -1.0; g0[5] = 1.0; g0[10] = r2;

@@ -71,6 +71,38 @@ GEON_D bool pole_pass(const FParams& P, float min_s, float min_r) {
 GEON_D float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 GEON_D float fsqrt(float x) { return __builtin_sqrtf(x); }
 
+// sin x and cos x in FP32 for |x| < 2^13 (else the library sincosf): the
+// Cody–Waite reduction x = k·π/2 + r with π/2 in three floats (each product
+// exact inside an FMA), Taylor polynomials of sin to r^9 and of cos to r^10
+// on |r| ≤ π/4 (truncation below 1e-9), and the quadrant k mod 4.
+#ifndef GEON_OPCOUNT   // the op-counting build tallies this as one sine and one cosine
+GEON_D void sincosf_cw(float x, float* s, float* c) {
+#pragma clang fp contract(fast)
+    if (!(fabsf(x) < 8192.0f)) {
+        sincosf(x, s, c);
+        return;
+    }
+    const float k = rintf(x * 0x1.45f306p-1f);                  // x·(2/π)
+    float r = __builtin_fmaf(-k, 0x1.921fb6p+0f, x);
+    r = __builtin_fmaf(-k, -0x1.777a5cp-25f, r);
+    r = __builtin_fmaf(-k, -0x1.ee59dap-50f, r);
+    const float r2 = r * r;
+    float ps = __builtin_fmaf(0x1.71de3ap-19f, r2, -0x1.a01a02p-13f);   //  1/9!, −1/7!
+    ps = __builtin_fmaf(ps, r2, 0x1.111112p-7f);                         //  1/5!
+    ps = __builtin_fmaf(ps, r2, -0x1.555556p-3f);                        // −1/3!
+    const float sr = __builtin_fmaf(r * r2, ps, r);
+    float pc = __builtin_fmaf(-0x1.27e4fcp-22f, r2, 0x1.a01a02p-16f);  // −1/10!, 1/8!
+    pc = __builtin_fmaf(pc, r2, -0x1.6c16c2p-10f);                       // −1/6!
+    pc = __builtin_fmaf(pc, r2, 0x1.555556p-5f);                         //  1/4!
+    const float cr = __builtin_fmaf(r2 * r2, pc, __builtin_fmaf(-0.5f, r2, 1.0f));
+    const int q = (int)k & 3;
+    const float sv = (q & 1) ? cr : sr;
+    const float cv = (q & 1) ? sr : cr;
+    *s = (q & 2) ? -sv : sv;
+    *c = ((q + 1) & 2) ? -cv : cv;
+}
+#endif
+
 // tanh via one exp: tanh(u) = 1 − 2/(e^{2u} + 1), exact saturation at ±1
 GEON_D float ftanh(float u) {
 #pragma clang fp contract(fast)
@@ -374,7 +406,7 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
     if constexpr (MF::uses_theta) {
-        sincosf(out[2], &s, &c);
+        sincosf_cw(out[2], &s, &c);
         MF::reproject(P, out, s);
         return fabsf(s);
     } else {

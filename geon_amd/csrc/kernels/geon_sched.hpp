@@ -164,7 +164,7 @@ struct Prec<MP, float> {
         return rk4_step_f<MF>(P, y, h, out, s, c, fr);
     }
     GEON_D static void trig(const float* y, float& s, float& c) {
-        if constexpr (MF::uses_theta) sincosf(y[2], &s, &c);
+        if constexpr (MF::uses_theta) sincosf_cw(y[2], &s, &c);
     }
     // a parked ray keeps its flat run, anchor included (planes 2–7), so the
     // positions it takes after resuming do not depend on where it was parked

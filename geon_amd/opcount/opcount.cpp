@@ -129,6 +129,7 @@ inline void sincosf(cnt_f a, cnt_f* s, cnt_f* c) {
     *s = cnt_f(::sinf(a.v));
     *c = cnt_f(::cosf(a.v));
 }
+inline void sincosf_cw(cnt_f a, cnt_f* s, cnt_f* c) { sincosf(a, s, c); }
 inline cnt_f cnt_rcpf(cnt_f a) { ++T.div; return cnt_f(1.0f / a.v); }
 inline cnt_f cnt_sqrtf(cnt_f a) { ++T.sqrt; return cnt_f(::sqrtf(a.v)); }
 inline cnt_f cnt_expf(cnt_f a) { ++T.tanh; return cnt_f(::expf(a.v)); }
