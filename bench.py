@@ -323,12 +323,12 @@ def serving(geon, ctx, torch, stream, outs, ev, seconds):
     out = {"workload": f"Poisson arrivals for {seconds} s at each rate: 80% renders side "
                        "log-uniform 32..2048 (random metric, camera r∈[10,50], escape 100, max 5000, "
                        "FP32-mixed), 20% bisections 64 rays × 40 halvings (M=1, r_obs∈[10,50], "
-                       "photon-sphere decisions); seed 7; 8 GiB admission budget, 512-step quanta",
+                       "photon-sphere decisions); seed 7; 8 GiB admission budget, 2048-step quanta",
            "rates": {}}
     plain_rate = None
     for rate in (50, 100, 200, 400):
         reqs = workloads.serving_stream(rate, int(rate * seconds))
-        srv = geon.Server(ctx, hbm_budget=8 << 30, quantum_steps=512)
+        srv = geon.Server(ctx, hbm_budget=8 << 30, quantum_steps=2048)
         tickets = []
         t_start = time.perf_counter()
         i = 0
