@@ -22,7 +22,7 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 
 @pytest.fixture(scope="module")
-def fn(tmp_path_factory):
+def dmlib(tmp_path_factory):
     if shutil.which("g++") is None:
         pytest.skip("no host compiler")
     so = str(tmp_path_factory.mktemp("scw") / "libscw.so")
@@ -34,12 +34,16 @@ def fn(tmp_path_factory):
     L = C.CDLL(so)
     L.geon_test_sincos_cw.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]
     L.geon_test_tanh_cw.argtypes = [C.c_void_p, C.c_long, C.c_void_p]
-    fn.lib = L
+    L.geon_test_sincosf_cw.argtypes = [C.c_void_p, C.c_long, C.c_void_p, C.c_void_p]
+    return L
 
+
+@pytest.fixture(scope="module")
+def fn(dmlib):
     def run(x):
         x = np.ascontiguousarray(x, dtype=np.float64)
         s, c = np.zeros_like(x), np.zeros_like(x)
-        L.geon_test_sincos_cw(x.ctypes.data, x.size, s.ctypes.data, c.ctypes.data)
+        dmlib.geon_test_sincos_cw(x.ctypes.data, x.size, s.ctypes.data, c.ctypes.data)
         return s, c
     return run
 
@@ -79,14 +83,14 @@ def test_sincos_cw_falls_back_beyond_range(fn):
     assert np.isnan(s[3:]).all() and np.isnan(c[3:]).all()
 
 
-def _tanh_cw(x):
+def _tanh_cw(lib, x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     out = np.zeros_like(x)
-    fn.lib.geon_test_tanh_cw(x.ctypes.data, x.size, out.ctypes.data)
+    lib.geon_test_tanh_cw(x.ctypes.data, x.size, out.ctypes.data)
     return out
 
 
-def test_tanh_cw_within_an_ulp(fn):
+def test_tanh_cw_within_an_ulp(dmlib):
     """Over [0, 25] (the shape function's arguments (r_s ± R)/s and beyond
     the saturation at 19.06), log-uniform down to 1e-20, next to every
     reduction boundary k·ln2/2, and negative arguments: within 0.75 ulp of
@@ -100,17 +104,49 @@ def test_tanh_cw_within_an_ulp(fn):
         np.concatenate([k * ln2 / 2 * (1 + rng.uniform(-1e-9, 1e-9, 40)) for k in range(1, 64)]),
         -rng.uniform(0.0, 25.0, 1000),
     ])
-    got = _tanh_cw(x)
+    got = _tanh_cw(dmlib, x)
     err = _ulp_err(got, x, mpmath.tanh)
     print(f"tanh_cw: max error {err.max():.3f} ulp, mean {err.mean():.3f} over {x.size} arguments")
     assert err.max() <= 0.75
 
 
-def test_tanh_cw_special_values(fn):
+def test_tanh_cw_special_values(dmlib):
     x = np.array([0.0, -0.0, math.inf, -math.inf, 22.0, 19.5, 700.0, -700.0, 1e-300, -5e-324,
                   2.0 ** -28, math.nan])
-    got = _tanh_cw(x)
+    got = _tanh_cw(dmlib, x)
     want = np.tanh(x)
     assert np.array_equal(got[:-1], want[:-1])
     assert np.array_equal(np.signbit(got[:-1]), np.signbit(want[:-1]))
     assert np.isnan(got[-1])
+
+
+def test_sincosf_cw_fp32(dmlib):
+    """The FP32 path's sincos (π/2 in three floats, Taylor to r^9 / r^10):
+    within 2 ulp of the correctly rounded FP32 sine and cosine over the
+    angles a ray's θ takes and the neighbourhoods of the multiples of π/2,
+    falling back to the library beyond 2^13."""
+    rng = np.random.default_rng(12)
+    x = np.concatenate([
+        rng.uniform(-10.0, 10.0, 3000), rng.uniform(0.0, math.pi, 3000),
+        np.concatenate([k * math.pi / 2 + np.sign(rng.uniform(-1, 1, 200)) *
+                        10.0 ** rng.uniform(-6, -1, 200) for k in range(-2, 5)]),
+        np.array([0.0, 1e-30, 5000.0, -7000.0]),
+    ]).astype(np.float32)
+    s, c = np.zeros_like(x), np.zeros_like(x)
+    dmlib.geon_test_sincosf_cw(x.ctypes.data, x.size, s.ctypes.data, c.ctypes.data)
+    mpmath.mp.dps = 30
+    def err(got, f):
+        out = []
+        for g, v in zip(got, x):
+            exact = f(mpmath.mpf(float(v)))
+            e32 = np.float32(float(exact))
+            ulp = float(np.spacing(np.abs(e32))) if e32 != 0 else 1e-45
+            out.append(abs(float(mpmath.mpf(float(g)) - exact)) / ulp)
+        return np.array(out)
+    es, ec = err(s, mpmath.sin), err(c, mpmath.cos)
+    print(f"sincosf_cw: max error sin {es.max():.3f} ulp, cos {ec.max():.3f} ulp")
+    assert es.max() <= 2.0 and ec.max() <= 2.0
+    big = np.array([9000.0, -1e6], dtype=np.float32)
+    s2, c2 = np.zeros_like(big), np.zeros_like(big)
+    dmlib.geon_test_sincosf_cw(big.ctypes.data, big.size, s2.ctypes.data, c2.ctypes.data)
+    assert np.allclose(s2, np.sin(big.astype(np.float64)), atol=1e-6)
