@@ -17,6 +17,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include <utility>
 
 #include "../geon_dev_types.h"
@@ -172,6 +173,31 @@ GEON_D double tanh_cw(double x) {
 // The same as a call: the fourteen tanh of an Alcubierre Γ, inlined and
 // interleaved, would push the ray kernels past 256 VGPRs.
 __device__ __attribute__((noinline)) double tanh_cw_call(double x) { return tanh_cw(x); }
+
+// sin and cos of θ = θ0 + d from s0 = sin θ0 and c0 = cos θ0, for
+// |d| ≤ 1/16 by rotation: sin d = d + d³·P(d²) and cos d − 1 = d²·Q(d²)
+// from their Taylor series to d^9 and d^10 (truncation below 1e-20), then
+// sin θ = s0 + (s0·(cos d − 1) + c0·sin d) and cos θ = c0 + (c0·(cos d − 1)
+// − s0·sin d): the corrections are small, so the result carries the error of
+// (s0, c0) plus one rounding. Larger |d| take sincos_cw(θ).
+GEON_D void sincos_rot(double th, double s0, double c0, double d, double* s, double* c) {
+    if (!(fabs(d) <= 0.0625)) {
+        sincos_cw(th, s, c);
+        return;
+    }
+    const double d2 = d * d;
+    double ps = __builtin_fma(0x1.71de3a556c734p-19, d2, sconst(-0x1.a01a01a01a01ap-13));  //  1/9!, −1/7!
+    ps = __builtin_fma(ps, d2, sconst(0x1.1111111111111p-7));                             //  1/5!
+    ps = __builtin_fma(ps, d2, sconst(-0x1.5555555555555p-3));                            // −1/3!
+    const double sd = __builtin_fma(d * d2, ps, d);
+    double pc = __builtin_fma(-0x1.27e4fb7789f5cp-22, d2, sconst(0x1.a01a01a01a01ap-16)); // −1/10!, 1/8!
+    pc = __builtin_fma(pc, d2, sconst(-0x1.6c16c16c16c17p-10));                           // −1/6!
+    pc = __builtin_fma(pc, d2, sconst(0x1.5555555555555p-5));                             //  1/4!
+    pc = __builtin_fma(pc, d2, -0.5);                                                     // −1/2!
+    const double cm = d2 * pc;
+    *s = s0 + __builtin_fma(s0, cm, c0 * sd);
+    *c = c0 + __builtin_fma(c0, cm, -(s0 * sd));
+}
 #endif
 
 // sin θ and cos θ of the spherical charts: sincos_cw on the GPU. The host
@@ -183,6 +209,17 @@ GEON_D void theta_sincos(double x, double* s, double* c) {
     sincos_cw(x, s, c);
 #else
     sincos(x, s, c);
+#endif
+}
+// sin θ and cos θ of an RK4 stage whose θ differs from the step's θ0 by the
+// stage increment: on the GPU by rotation from (sin θ0, cos θ0); the host
+// build evaluates them directly like theta_sincos.
+GEON_D void stage_sincos(double th, double th0, double s0, double c0, double* s, double* c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    sincos_rot(th, s0, c0, th - th0, s, c);
+#else
+    (void)th0; (void)s0; (void)c0;
+    sincos(th, s, c);
 #endif
 }
 
@@ -499,10 +536,15 @@ struct MorrisThorne {
     // ~1e-11 relative (its FD quotient amplifies by 1/h). Every other entry
     // is the value the generic loop produces.
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
-        const double l = x[1], th = x[2], h = P.fd_h, two_h = P.fd_two_h;
-        const double b2 = P.b0 * P.b0;
         double st, ct;
-        theta_sincos(th, &st, &ct);
+        theta_sincos(x[2], &st, &ct);
+        christoffel_sc(P, x, st, ct, c);
+    }
+    // the same with st = sin θ and ct = cos θ given
+    GEON_D static void christoffel_sc(const geon_dmetric& P, const double* x, double st, double ct,
+                                      double* c) {
+        const double l = x[1], h = P.fd_h, two_h = P.fd_two_h;
+        const double b2 = P.b0 * P.b0;
         const double r2 = b2 + l * l;
         double g0[16], gi[16], dgv[64];
         g0[0] = -1.0; g0[5] = 1.0; g0[10] = r2;
@@ -525,6 +567,14 @@ struct MorrisThorne {
     GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
         double c[64];
         christoffel(P, y, c);
+        contract<MorrisThorne>(c, y + 4, dp);
+    }
+    // rk4_step hands each stage sin θ and cos θ (stage_sincos)
+    static constexpr bool theta_stages = true;
+    GEON_D static void accel_sc(const geon_dmetric& P, const double* y, double* dp, double st,
+                                double ct) {
+        double c[64];
+        christoffel_sc(P, y, st, ct, c);
         contract<MorrisThorne>(c, y + 4, dp);
     }
     static constexpr bool dep(int d) { return d == 1 || d == 2; }
@@ -557,7 +607,7 @@ GEON_D double shape_tanh(double x) {
 }
 
 struct Alcubierre {
-    static constexpr int flat_test = 2;   // flat steps decided per step by flat()
+    static constexpr int flat_test = 2;   // flat steps decided by flat_steps()
     static constexpr bool g_nz(int m, int n) {
         return m == n || (m == 0 && n == 1) || (m == 1 && n == 0);
     }
@@ -675,6 +725,15 @@ GEON_D void rk4_step_flat(const double* y, double h, double* out) {
 
 // k_i = (p_i, dp_i); the combination ((k1 + 2k2) + 2k3) + k4 is accumulated
 // in that order. Metrics with flat regions take the folded step there.
+// Metrics whose Γ takes sin θ and cos θ as arguments (theta_stages) get
+// them from rk4_step: theta_sincos at the step's θ, stage_sincos at the
+// stages'.
+template <class MP, class = void>
+struct ThetaStages : std::false_type {};
+template <class MP>
+struct ThetaStages<MP, std::void_t<decltype(MP::theta_stages)>>
+    : std::integral_constant<bool, MP::theta_stages> {};
+
 // Steps of a ray still known to be flat from its last flatness test
 // (metrics whose flat regions are tested per step); 0: test again.
 struct FlatRun {
@@ -701,7 +760,22 @@ GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* o
     }
     double acc[8], yt[8], dp[4];
     const double hh = 0.5 * h;
-    MP::accel(P, y, dp);
+    double s0 = 0.0, c0 = 0.0;
+    auto stage = [&](const double* ys) {
+        if constexpr (ThetaStages<MP>::value) {
+            double st, ct;
+            stage_sincos(ys[2], y[2], s0, c0, &st, &ct);
+            MP::accel_sc(P, ys, dp, st, ct);
+        } else {
+            MP::accel(P, ys, dp);
+        }
+    };
+    if constexpr (ThetaStages<MP>::value) {
+        theta_sincos(y[2], &s0, &c0);
+        MP::accel_sc(P, y, dp, s0, c0);
+    } else {
+        MP::accel(P, y, dp);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         acc[i] = y[4 + i];
@@ -709,7 +783,7 @@ GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* o
         yt[i] = y[i] + hh * y[4 + i];
         yt[4 + i] = y[4 + i] + hh * dp[i];
     }
-    MP::accel(P, yt, dp);
+    stage(yt);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const double kx = yt[4 + i];
@@ -718,7 +792,7 @@ GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* o
         yt[i] = y[i] + hh * kx;
         yt[4 + i] = y[4 + i] + hh * dp[i];
     }
-    MP::accel(P, yt, dp);
+    stage(yt);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const double kx = yt[4 + i];
@@ -727,7 +801,7 @@ GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* o
         yt[i] = y[i] + h * kx;
         yt[4 + i] = y[4 + i] + h * dp[i];
     }
-    MP::accel(P, yt, dp);
+    stage(yt);
     const double sixth = h / 6.0;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
