@@ -136,19 +136,24 @@ struct MinkowskiF {
 struct SchwarzschildF {
     static constexpr bool uses_theta = true;
     static constexpr int flat_test = 0;
+    // 1/r, 1/(r − 2M) and cot θ from one reciprocal of r·(r − 2M)·sin θ
+    // (reciprocals issue at a quarter of the VALU rate)
     GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
 #pragma clang fp contract(fast)
         const float r = y[1];
         const float pt = y[4], pr = y[5], pth = y[6], pph = y[7];
-        const float inv_r = frcp(r);
-        const float m_r = P.M * inv_r;
-        const float f = 1.0f - 2.0f * m_r;
-        const float inv_f = frcp(f);
-        const float a = m_r * inv_r;                 // M/r²
-        const float gtr = a * inv_f;                 // Γ^t_tr
+        const bool pole = fabsf(s) < 1e-8f;
+        const float se = pole ? 1.0f : s;
+        const float rf = r - 2.0f * P.M;             // r − 2M = r·f
+        const float rr = r * rf;
+        const float q = frcp(rr * se);
+        const float inv_r = rf * se * q;
+        const float inv_rr = se * q;                 // 1/(r(r − 2M))
+        const float f = rf * inv_r;
+        const float a = P.M * inv_r * inv_r;         // M/r²
+        const float gtr = P.M * inv_rr;              // Γ^t_tr = M/(r²f)
         const float grtt = a * f;                    // Γ^r_tt
-        const float rf = r * f;                      // r − 2M
-        const float cot = (fabsf(s) < 1e-8f) ? 0.0f : c * frcp(s);
+        const float cot = pole ? 0.0f : c * rr * q;
         const float pph2 = pph * pph;
         dp[0] = -2.0f * gtr * pt * pr;
         dp[1] = -(grtt * pt * pt - gtr * pr * pr) + rf * (pth * pth + s * s * pph2);
@@ -158,9 +163,10 @@ struct SchwarzschildF {
     GEON_D static void reproject(const FParams& P, float* y, float s) {
 #pragma clang fp contract(fast)
         const float r = y[1];
-        const float f = 1.0f - 2.0f * P.M * frcp(r);
-        const float sp = y[5] * y[5] * frcp(f) + r * r * (y[6] * y[6] + s * s * y[7] * y[7]);
-        if (f > 0.0f && sp >= 0.0f) y[4] = fsqrt(sp * frcp(f));
+        const float rf = r - 2.0f * P.M;
+        const float inv_f = r * r * frcp(r * rf);    // 1/f = r/(r − 2M)
+        const float sp = y[5] * y[5] * inv_f + r * r * (y[6] * y[6] + s * s * y[7] * y[7]);
+        if (rf > 0.0f && sp >= 0.0f) y[4] = fsqrt(sp * inv_f);
     }
     GEON_D static int terminate(const FParams& P, const float* x) {
         const float r = x[1];
@@ -188,9 +194,13 @@ struct MorrisThorneF {
 #pragma clang fp contract(fast)
         const float l = y[1];
         const float pl = y[5], pth = y[6], pph = y[7];
-        const float inv_r2 = frcp(P.b0sq + l * l);
+        const bool pole = fabsf(s) < 1e-8f;
+        const float se = pole ? 1.0f : s;
+        const float r2 = P.b0sq + l * l;
+        const float q = frcp(r2 * se);               // 1/r² and cot θ from one reciprocal
+        const float inv_r2 = se * q;
         const float lr = l * inv_r2;                 // Γ^θ_lθ = Γ^φ_lφ
-        const float cot = (fabsf(s) < 1e-8f) ? 0.0f : c * frcp(s);
+        const float cot = pole ? 0.0f : c * r2 * q;
         const float pph2 = pph * pph;
         dp[0] = 0.0f;
         dp[1] = l * (pth * pth + s * s * pph2);      // −(Γ^l_θθ pθ² + Γ^l_φφ pφ²)
