@@ -300,38 +300,33 @@ struct FD {
         });
     }
 
+    // The first σ whose term enters Γ^μ_αβ's sum.
+    static constexpr int first_sigma(int mu, int a, int b) {
+        for (int sg = 0; sg < 4; ++sg)
+            if (MP::ginv_nz(mu, sg) && S_nz(sg, a, b)) return sg;
+        return 4;
+    }
+
     // Γ^μ_αβ = 0.5 Σ_σ g^μσ S(σ,α,β) (metric/mod.rs:55-66); Γ^μ_βα equals
     // Γ^μ_αβ bit for bit (S is a commutative sum of the same two terms
     // minus a symmetric entry) and is copied.
+    // SIGNED: each sum starts from 0.0 as the reference's does, which turns
+    // a −0 sum into +0. Ray steps pass false and start from the first term
+    // (one add less per entry): there Γ only enters contract(), whose own sum
+    // from 0.0 gives the same value for a +0 and a −0 term, so dp does not
+    // change. Curvature and the exported Γ keep the reference's zeros.
+    template <bool SIGNED = true>
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* gam) {
         double g0[16], gi[16], dgv[64];
         MP::g(P, x, g0);
         MP::ginv(P, g0, gi);
         dg(P, x, dgv);
-        sfor<4>([&](auto Mu) {
-            sfor<4>([&](auto A) {
-                sfor<4>([&](auto B) {
-                    constexpr int mu = CI(Mu), a = CI(A), b = CI(B);
-                    if constexpr (gam_nz(mu, a, b)) {
-                        if constexpr (b < a && gam_nz(mu, b, a)) {
-                            gam[mu * 16 + a * 4 + b] = gam[mu * 16 + b * 4 + a];
-                        } else {
-                            double s = 0.0;
-                            sfor<4>([&](auto Sg) {
-                                constexpr int sg = CI(Sg);
-                                if constexpr (MP::ginv_nz(mu, sg) && S_nz(sg, a, b))
-                                    s += gi[mu * 4 + sg] * S<sg, a, b>(dgv);
-                            });
-                            gam[mu * 16 + a * 4 + b] = 0.5 * s;
-                        }
-                    }
-                });
-            });
-        });
+        assemble<SIGNED>(gi, dgv, gam);
     }
 
-    // The same assembly from a g⁻¹ and ∂g the caller computed itself (a
-    // metric whose ∂g is written out with shared subexpressions).
+    // The assembly from a g⁻¹ and ∂g (christoffel above, or a metric whose
+    // ∂g is written out with shared subexpressions).
+    template <bool SIGNED = true>
     GEON_D static void assemble(const double* gi, const double* dgv, double* gam) {
         sfor<4>([&](auto Mu) {
             sfor<4>([&](auto A) {
@@ -344,8 +339,11 @@ struct FD {
                             double s = 0.0;
                             sfor<4>([&](auto Sg) {
                                 constexpr int sg = CI(Sg);
-                                if constexpr (MP::ginv_nz(mu, sg) && S_nz(sg, a, b))
-                                    s += gi[mu * 4 + sg] * S<sg, a, b>(dgv);
+                                if constexpr (MP::ginv_nz(mu, sg) && S_nz(sg, a, b)) {
+                                    const double t = gi[mu * 4 + sg] * S<sg, a, b>(dgv);
+                                    if constexpr (!SIGNED && sg == first_sigma(mu, a, b)) s = t;
+                                    else s += t;
+                                }
                             });
                             gam[mu * 16 + a * 4 + b] = 0.5 * s;
                         }
@@ -511,11 +509,15 @@ struct MorrisThorne {
         const double n5 = m0 * m10 * m15, n10 = m0 * m5 * m15, n15 = m0 * m5 * m10;
         const double ad = fabs(det);
         if (__builtin_expect(ad > 0x1p-500 && ad < 0x1p500, 1)) {
-            // one division serves the four quotients; with det ≠ 0 every
-            // numerator is a nonzero product (g_θθ = b0² + l² > 0, g_φφ ≠ 0)
+            // g_tt = −1 and g_ll = 1 (g() sets them), so c00 = (1·m10)·m15 and
+            // n5 = (−1·m10)·m15 both round the one product m10·m15 and det =
+            // −c00 = n5 bit for bit: c00/det is exactly −1 and n5/det exactly
+            // 1 for every finite nonzero det. One division serves the other
+            // two quotients; with det ≠ 0 both numerators are nonzero
+            // (g_θθ = b0² + l² > 0, g_φφ ≠ 0).
             const double y = 1.0 / det;
-            gi[0] = div_rn_pz(c00, det, y);
-            gi[5] = div_rn_pz(n5, det, y);
+            gi[0] = -1.0;
+            gi[5] = 1.0;
             gi[10] = div_rn_pz(n10, det, y);
             gi[15] = div_rn_pz(n15, det, y);
         } else {
@@ -535,12 +537,15 @@ struct MorrisThorne {
     // same size as one libm against another, so ∂_θ g_φφ moves by
     // ~1e-11 relative (its FD quotient amplifies by 1/h). Every other entry
     // is the value the generic loop produces.
+    // SIGNED: as FD::assemble (false in ray steps)
+    template <bool SIGNED = true>
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
         double st, ct;
         theta_sincos(x[2], &st, &ct);
-        christoffel_sc(P, x, st, ct, c);
+        christoffel_sc<SIGNED>(P, x, st, ct, c);
     }
     // the same with st = sin θ and ct = cos θ given
+    template <bool SIGNED = true>
     GEON_D static void christoffel_sc(const geon_dmetric& P, const double* x, double st, double ct,
                                       double* c) {
         const double l = x[1], h = P.fd_h, two_h = P.fd_two_h;
@@ -562,11 +567,11 @@ struct MorrisThorne {
         const double sp = st * P.fd_cos_h + ct * P.fd_sin_h;   // sin(θ + h)
         const double sm = st * P.fd_cos_h - ct * P.fd_sin_h;   // sin(θ − h)
         dgv[2 * 16 + 3 * 4 + 3] = div_rn_pz(r2 * sp * sp - r2 * sm * sm, two_h, i2h);
-        FD<MorrisThorne>::assemble(gi, dgv, c);
+        FD<MorrisThorne>::assemble<SIGNED>(gi, dgv, c);
     }
     GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
         double c[64];
-        christoffel(P, y, c);
+        christoffel<false>(P, y, c);
         contract<MorrisThorne>(c, y + 4, dp);
     }
     // rk4_step hands each stage sin θ and cos θ (stage_sincos)
@@ -574,7 +579,7 @@ struct MorrisThorne {
     GEON_D static void accel_sc(const geon_dmetric& P, const double* y, double* dp, double st,
                                 double ct) {
         double c[64];
-        christoffel_sc(P, y, st, ct, c);
+        christoffel_sc<false>(P, y, st, ct, c);
         contract<MorrisThorne>(c, y + 4, dp);
     }
     static constexpr bool dep(int d) { return d == 1 || d == 2; }
@@ -657,8 +662,9 @@ struct Alcubierre {
         gi[10] = 1.0;
         gi[15] = 1.0;
     }
+    template <bool SIGNED = true>
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
-        FD<Alcubierre>::christoffel(P, x, c);
+        FD<Alcubierre>::christoffel<SIGNED>(P, x, c);
     }
     static constexpr bool dep(int d) { return d >= 1; }
     GEON_D static void ginv_at(const geon_dmetric& P, const double* x, double* gi) {
@@ -674,7 +680,7 @@ struct Alcubierre {
             return;
         }
         double c[64];
-        christoffel(P, y, c);
+        christoffel<false>(P, y, c);
         contract<Alcubierre>(c, y + 4, dp);
     }
     GEON_D static int terminate(const geon_dmetric& P, const double* x) {
@@ -740,24 +746,9 @@ struct FlatRun {
     uint32_t left = 0;
 };
 
+// The full step (Γ evaluated at the four stages).
 template <class MP>
-GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out, FlatRun& fr) {
-    if constexpr (MP::flat_test == 1) {
-        rk4_step_flat(y, h, out);
-        return;
-    } else if constexpr (MP::flat_test == 2) {
-        if (fr.left > 0) {
-            --fr.left;
-            rk4_step_flat(y, h, out);
-            return;
-        }
-        const uint32_t k = MP::flat_steps(P, y, h);
-        if (k > 0) {
-            fr.left = k - 1;
-            rk4_step_flat(y, h, out);
-            return;
-        }
-    }
+GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, double* out) {
     double acc[8], yt[8], dp[4];
     const double hh = 0.5 * h;
     double s0 = 0.0, c0 = 0.0;
@@ -813,6 +804,27 @@ GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* o
 }
 
 template <class MP>
+GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out, FlatRun& fr) {
+    if constexpr (MP::flat_test == 1) {
+        rk4_step_flat(y, h, out);
+        return;
+    } else if constexpr (MP::flat_test == 2) {
+        if (fr.left > 0) {
+            --fr.left;
+            rk4_step_flat(y, h, out);
+            return;
+        }
+        const uint32_t k = MP::flat_steps(P, y, h);
+        if (k > 0) {
+            fr.left = k - 1;
+            rk4_step_flat(y, h, out);
+            return;
+        }
+    }
+    rk4_step_full<MP>(P, y, h, out);
+}
+
+template <class MP>
 GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out) {
     FlatRun fr;
     rk4_step<MP>(P, y, h, out, fr);
@@ -857,25 +869,104 @@ GEON_D void direction(const double* y, double* d) {
     }
 }
 
+// ---- flat spans ---------------------------------------------------------------
+// Along a run of flat steps (Γ exactly zero at every stage) the folded step
+// leaves the momentum as it is — p + (h/6)·(−0) is p, except that with h < 0
+// a −0 component becomes +0 once, which changes no increment below — and
+// adds the same increment δ_i = (h/6)·acc_i, acc_i as in rk4_step_flat, to
+// every position coordinate: x_{j+1} = x_j + δ with the rounding of the full
+// step. Where the positions of the next m steps provably stay inside the
+// escape sphere, neither the escape test before each of those steps nor the
+// finiteness guard after it can fire, so the m steps are m additions per
+// coordinate with the same bits as m checked steps.
+//
+// The bound: each step moves the spatial position by |δ_s|(1 + 2^-52) plus
+// at most 2^-53 of |x| of rounding, so after j ≤ 65536 steps |x_j| ≤ r0 +
+// j·|δ_s|(1 + 1e-15) + 1.5e-11·R. With j·|δ_s|(1 + 1e-9) ≤ R(1 − 1e-9) −
+// r0(1 + 1e-9) (R = √esc², r0 = |x_0|, both as computed, each within ulps)
+// that stays below R(1 − 9e-10), and the computed r² below esc²; t grows by
+// |δ_0| a step and stays far from overflow. Returns the number of steps taken
+// (0: none guaranteed, take a checked step).
+GEON_D uint32_t flat_span_advance(double* y, double h, double esc2, uint32_t cap) {
+    const double hh = 0.5 * h;
+    const double zh = hh * -0.0, zf = h * -0.0;
+    const double sixth = h / 6.0;
+    const double zs = sixth * -0.0;
+    double d[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double p = y[4 + i];
+        const double k23 = p + zh;
+        const double t = 2.0 * k23;
+        const double acc = ((p + t) + t) + (p + zf);
+        d[i] = sixth * acc;
+    }
+    const double r0 = sqrt(y[1] * y[1] + y[2] * y[2] + y[3] * y[3]);
+    const double ds = sqrt(d[1] * d[1] + d[2] * d[2] + d[3] * d[3]);
+    const double avail = sqrt(esc2) * (1.0 - 1e-9) - r0 * (1.0 + 1e-9);
+    const double mj = avail / (ds * (1.0 + 1e-9));   // +inf for δ_s = 0
+    if (!(avail > 0.0 && avail < 1e300) || !(mj >= 1.0) ||
+        !(fabs(y[0]) + 131072.0 * fabs(d[0]) < 1e300))
+        return 0;
+    const uint32_t m = mj >= (double)cap ? cap : (uint32_t)mj;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[4 + i] = y[4 + i] + zs;
+    double x0 = y[0], x1 = y[1], x2 = y[2], x3 = y[3];
+    for (uint32_t j = 0; j < m; ++j) {
+        x0 = x0 + d[0];
+        x1 = x1 + d[1];
+        x2 = x2 + d[2];
+        x3 = x3 + d[3];
+    }
+    y[0] = x0; y[1] = x1; y[2] = x2; y[3] = x3;
+    return m;
+}
+
 // trace (geodesic.rs:62-117): terminate first, step, NaN guard keeps the
-// pre-step state, max_steps → MAXSTEPS.
+// pre-step state, max_steps → MAXSTEPS. Metrics with flat regions (both
+// escape on x² + y² + z² > esc²) take runs of flat steps as spans.
 template <class MP>
 GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps, uint32_t* steps) {
     FlatRun fr;
-    for (uint32_t n = 0; n < max_steps; ++n) {
+    uint32_t n = 0;
+    while (n < max_steps) {
         const int t = MP::terminate(P, y);
         if (t != RUNNING) {
             *steps = n;
             return t;
         }
+        if constexpr (MP::flat_test != 0) {
+            // flat steps known from this one on (flat_test 1: every step)
+            uint32_t run = 65536u;
+            if constexpr (MP::flat_test == 2) {
+                run = fr.left;
+                if (run == 0) run = MP::flat_steps(P, y, h);
+                fr.left = run;   // rk4_step below takes a flat step while > 0
+            }
+            if (run > 0) {
+                const uint32_t cap = run < max_steps - n ? run : max_steps - n;
+                const uint32_t m = flat_span_advance(y, h, P.esc2, cap);
+                if (m > 0) {
+                    if constexpr (MP::flat_test == 2) fr.left = run - m;
+                    n += m;
+                    continue;
+                }
+            }
+        }
         double yn[8];
-        rk4_step<MP>(P, y, h, yn, fr);
+        if constexpr (MP::flat_test == 2) {
+            if (fr.left == 0) rk4_step_full<MP>(P, y, h, yn);   // tested above: not flat
+            else rk4_step<MP>(P, y, h, yn, fr);
+        } else {
+            rk4_step<MP>(P, y, h, yn, fr);
+        }
         if (!finite8(yn)) {
             *steps = n;
             return DIVERGED;
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) y[i] = yn[i];
+        ++n;
     }
     *steps = max_steps;
     return MAXSTEPS;

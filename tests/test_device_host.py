@@ -5,7 +5,8 @@ The device formulation skips structural zeros, specialises the 4×4 inverse
 to each metric's sparsity and folds zero-Γ steps; none of that may change a
 result. On the host, with glibc's transcendentals on both sides, Minkowski,
 Schwarzschild and Alcubierre traces and Γ therefore equal the oracle's
-exactly. Morris–Thorne takes sin(θ ± h) by angle addition and sin/cos θ from
+exactly, runs of flat steps taken as spans of position additions
+included. Morris–Thorne takes sin(θ ± h) by angle addition and sin/cos θ from
 its own Cody–Waite routine (both within ulps of libm), so it is held to
 closeness instead. The Alcubierre curvature team (geon_curv.hpp: 31 shared
 metric points, seven Γ on seven lanes, outputs spread over eight lanes),
@@ -80,6 +81,36 @@ def test_traces_match_oracle(hd, O, name, kw, obs, max_steps):
         assert np.abs(Y - ref["y"]).max() < 1e-8
     else:
         assert np.array_equal(Y, ref["y"])
+
+
+# Runs of flat steps taken as spans of position additions: long runs, escape
+# spheres just outside and inside the camera, a negative step with ±0
+# momentum components, an odd step size, and runs cut by max_steps.
+SPAN_CASES = [
+    ("minkowski", dict(escape_radius=1e4), [0, 20, 3, -1], 0.05, 60000, None),
+    ("minkowski", dict(escape_radius=50.0), [0, 20, 0, 0], -0.07, 3000, "zeros"),
+    ("minkowski", dict(escape_radius=20.05), [0, 20, 0, 0], 0.05, 3000, None),
+    ("minkowski", dict(escape_radius=10.0), [0, 20, 0, 0], 0.05, 3000, None),
+    ("alcubierre", dict(), [0, -10, 0, 0], 0.05, 20000, None),
+    ("alcubierre", dict(escape_radius=40.0), [0, 10, 0.5, 0], -0.05, 8000, None),
+    ("alcubierre", dict(escape_radius=25.0, ship_velocity=2.0), [0, 6, 0, 0], 0.0371, 5000, None),
+]
+
+
+@pytest.mark.parametrize("name,kw,obs,h,max_steps,zeros", SPAN_CASES,
+                         ids=[f"{c[0]}-{i}" for i, c in enumerate(SPAN_CASES)])
+def test_flat_spans_match_oracle(hd, O, name, kw, obs, h, max_steps, zeros):
+    kind = {"minkowski": O.MINKOWSKI, "alcubierre": O.ALCUBIERRE}[name]
+    m = O.metric(kind, **kw)
+    y0 = np.array(O.camera_rays(m, O.camera(obs, 70.0, 12, 12)))
+    if zeros:
+        y0[:5, 6] = -0.0
+        y0[5:10, 7] = 0.0
+    ref = O.trace_rays(m, y0, h, max_steps)
+    cls, steps, Y = _trace(hd, m, y0, h, max_steps)
+    assert np.array_equal(cls, ref["cls"]) and np.array_equal(steps, ref["steps"])
+    # the same bits, signs of zero included
+    assert np.array_equal(Y.view(np.uint64), np.asarray(ref["y"]).view(np.uint64))
 
 
 def test_christoffel_matches_oracle(hd, O):
