@@ -744,6 +744,7 @@ struct ThetaStages<MP, std::void_t<decltype(MP::theta_stages)>>
 // (metrics whose flat regions are tested per step); 0: test again.
 struct FlatRun {
     uint32_t left = 0;
+    bool nf = false;   // the flatness test of this step already ran and failed
 };
 
 // The full step (Γ evaluated at the four stages).
@@ -814,11 +815,15 @@ GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* o
             rk4_step_flat(y, h, out);
             return;
         }
-        const uint32_t k = MP::flat_steps(P, y, h);
-        if (k > 0) {
-            fr.left = k - 1;
-            rk4_step_flat(y, h, out);
-            return;
+        if (fr.nf) {
+            fr.nf = false;
+        } else {
+            const uint32_t k = MP::flat_steps(P, y, h);
+            if (k > 0) {
+                fr.left = k - 1;
+                rk4_step_flat(y, h, out);
+                return;
+            }
         }
     }
     rk4_step_full<MP>(P, y, h, out);
@@ -922,9 +927,31 @@ GEON_D uint32_t flat_span_advance(double* y, double h, double esc2, uint32_t cap
     return m;
 }
 
+// The flat span for a state whose escape test just passed, at most `cap`
+// steps: advances y and the run, returns the steps taken (0: take an
+// ordinary step, which then knows the flatness test's outcome). Metrics with
+// flat regions escape on x² + y² + z² > esc².
+template <class MP>
+GEON_D uint32_t flat_span(const geon_dmetric& P, double* y, double h, FlatRun& fr, uint32_t cap) {
+    uint32_t run = 65536u;   // flat steps known from this one on (flat_test 1: all)
+    if constexpr (MP::flat_test == 2) {
+        run = fr.left;
+        if (run == 0) {
+            run = MP::flat_steps(P, y, h);
+            if (run == 0) {
+                fr.nf = true;
+                return 0;
+            }
+        }
+        fr.left = run;   // an ordinary step takes a flat step while > 0
+    }
+    const uint32_t m = flat_span_advance(y, h, P.esc2, run < cap ? run : cap);
+    if constexpr (MP::flat_test == 2) fr.left = run - m;
+    return m;
+}
+
 // trace (geodesic.rs:62-117): terminate first, step, NaN guard keeps the
-// pre-step state, max_steps → MAXSTEPS. Metrics with flat regions (both
-// escape on x² + y² + z² > esc²) take runs of flat steps as spans.
+// pre-step state, max_steps → MAXSTEPS. Runs of flat steps go as spans.
 template <class MP>
 GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps, uint32_t* steps) {
     FlatRun fr;
@@ -936,30 +963,14 @@ GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps,
             return t;
         }
         if constexpr (MP::flat_test != 0) {
-            // flat steps known from this one on (flat_test 1: every step)
-            uint32_t run = 65536u;
-            if constexpr (MP::flat_test == 2) {
-                run = fr.left;
-                if (run == 0) run = MP::flat_steps(P, y, h);
-                fr.left = run;   // rk4_step below takes a flat step while > 0
-            }
-            if (run > 0) {
-                const uint32_t cap = run < max_steps - n ? run : max_steps - n;
-                const uint32_t m = flat_span_advance(y, h, P.esc2, cap);
-                if (m > 0) {
-                    if constexpr (MP::flat_test == 2) fr.left = run - m;
-                    n += m;
-                    continue;
-                }
+            const uint32_t m = flat_span<MP>(P, y, h, fr, max_steps - n);
+            if (m > 0) {
+                n += m;
+                continue;
             }
         }
         double yn[8];
-        if constexpr (MP::flat_test == 2) {
-            if (fr.left == 0) rk4_step_full<MP>(P, y, h, yn);   // tested above: not flat
-            else rk4_step<MP>(P, y, h, yn, fr);
-        } else {
-            rk4_step<MP>(P, y, h, yn, fr);
-        }
+        rk4_step<MP>(P, y, h, yn, fr);
         if (!finite8(yn)) {
             *steps = n;
             return DIVERGED;

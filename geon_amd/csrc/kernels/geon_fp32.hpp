@@ -337,6 +337,7 @@ struct FlatRunF {
     uint32_t left = 0;
     uint32_t j = 0;
     float a[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    bool nf = false;   // the flatness test of this step already ran and failed
 };
 
 template <class MF>
@@ -350,6 +351,9 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
         if constexpr (MF::flat_test == 2) {
             if (fr.left > 0) {
                 --fr.left;
+            } else if (fr.nf) {
+                fr.nf = false;
+                flat = false;
             } else {
                 const uint32_t k = MF::flat_steps(P, y, h);
                 fr.left = k > 0 ? k - 1 : 0;
@@ -429,6 +433,55 @@ template <class MF>
 GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c) {
     FlatRunF fr;
     return rk4_step_f<MF>(P, y, h, out, s, c, fr);
+}
+
+// A span of flat steps for a state whose escape test just passed, at most
+// `cap` steps: the anchored step above puts step j of a run at a + (j·h)·p
+// whatever the steps before it did, so m steps are one evaluation at
+// j + m — where every position up to it provably stays inside the escape
+// sphere, so that no escape test or finiteness guard in between could have
+// fired. Bound: |fl(j·h)| ≤ j|h|(1 + 2^-24) and the FMA rounds once, so with
+// j|h||p_s|(1 + 1e-5) ≤ R(1 − 1e-5) − |a|(1 + 1e-5), R = √esc², every
+// position is within R(1 − 9e-6) and its r² below esc². Momenta stay as
+// they are. Returns the steps taken (0: take an ordinary step, which then
+// knows the flatness test's outcome).
+template <class MF>
+GEON_D uint32_t flat_span_f(const FParams& P, float* y, float h, FlatRunF& fr, uint32_t cap) {
+#pragma clang fp contract(fast)
+    uint32_t run = 1u << 22;
+    if constexpr (MF::flat_test == 2) {
+        run = fr.left;
+        if (run == 0) {
+            run = MF::flat_steps(P, y, h);
+            if (run == 0) {
+                fr.nf = true;
+                return 0;
+            }
+            fr.j = 0;   // a new run: anchored at this state
+        }
+        fr.left = run;  // an ordinary step takes a flat step while > 0
+    }
+    if (fr.j == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) fr.a[i] = y[i];
+    }
+    const float ra = fsqrt(fr.a[1] * fr.a[1] + fr.a[2] * fr.a[2] + fr.a[3] * fr.a[3]);
+    const float ps = fsqrt(y[5] * y[5] + y[6] * y[6] + y[7] * y[7]);
+    const float avail = fsqrt(P.esc2) * (1.0f - 1e-5f) - ra * (1.0f + 1e-5f);
+    // j stays below 2^22, where (float)j is exact
+    const float jlim = fminf(avail / (fabsf(h) * ps * (1.0f + 1e-5f)), 4194304.0f);
+    if (!(avail > 0.0f && avail < 1e30f) || !(jlim >= (float)fr.j + 1.0f) ||
+        !(fabsf(fr.a[0]) + 8388608.0f * fabsf(h * y[4]) < 1e30f))
+        return 0;
+    uint32_t m = (uint32_t)jlim - fr.j;
+    if (m > run) m = run;
+    if (m > cap) m = cap;
+    fr.j += m;
+    if constexpr (MF::flat_test == 2) fr.left = run - m;
+    const float jh = (float)fr.j * h;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) y[i] = fr.a[i] + jh * y[4 + i];
+    return m;
 }
 
 GEON_D bool finite8f(const float* y) {

@@ -139,6 +139,10 @@ struct Prec<MP, double> {
         rk4_step<MP>(P, y, h, out, fr);
         return 1.0f;
     }
+    GEON_D static uint32_t span(const Params& P, double* y, double h, FlatRun& fr, uint32_t cap) {
+        if constexpr (MP::flat_test != 0) return flat_span<MP>(P, y, h, fr, cap);
+        else return 0;
+    }
     GEON_D static void trig(const double*, float&, float&) {}
     // a parked ray keeps its flat run (plane 2 of the slot's auxiliary words)
     GEON_D static void park_run(const Run& fr, float* aux, size_t n, uint32_t s) {
@@ -162,6 +166,16 @@ struct Prec<MP, float> {
     GEON_D static float step(const Params& P, const float* y, float h, float* out, float& s, float& c,
                              FlatRunF& fr) {
         return rk4_step_f<MF>(P, y, h, out, s, c, fr);
+    }
+    // flat spans leave min_s and min_r alone: a flat step returns 1 and these
+    // metrics' radius() is 1e30, so the trackers could not move
+    GEON_D static uint32_t span(const Params& P, float* y, float h, FlatRunF& fr, uint32_t cap) {
+        if constexpr (MF::flat_test != 0) {
+            static_assert(!MF::uses_theta, "flat spans assume no θ trackers");
+            return flat_span_f<MF>(P, y, h, fr, cap);
+        } else {
+            return 0;
+        }
     }
     GEON_D static void trig(const float* y, float& s, float& c) {
         if constexpr (MF::uses_theta) sincosf_cw(y[2], &s, &c);
@@ -337,6 +351,14 @@ __device__ void sched_run(const geon_sched_args& A) {
                 if (t != RUNNING) {
                     finish(r, t);
                     continue;
+                }
+                if constexpr (MP::flat_test != 0) {
+                    const uint32_t m = PR::span(P[r], y[r], h[r], fr[r], maxs[r] - n[r]);
+                    if (m > 0) {
+                        n[r] += m;
+                        executed += m;
+                        continue;
+                    }
                 }
                 T yn[8];
                 float s2 = sn[r], c2 = cs[r];
