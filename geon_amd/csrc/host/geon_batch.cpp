@@ -419,13 +419,16 @@ int geon_batch_totals(geon_batch_t* b, uint64_t* ray_steps, uint64_t* fp64_rays,
 namespace {
 
 // Calibration frame of each metric kind: small enough to time in milliseconds,
-// ragged enough (captured, escaped and long rays) to exercise refill.
+// ragged enough (captured, escaped and long rays) to exercise refill, and
+// large enough (410k rays) to fill the persistent grid (256 CUs × 4 SIMDs ×
+// 4 waves × 64 lanes = 262k slots at one ray per lane), so that a shape wins
+// for the load it will carry and not for a half-empty chip.
 void calibration(int kind, geon_metric_t* m, geon_camera_t* cam, geon_render_cfg_t* cfg) {
     geon_metric_default(kind, m);
     memset(cam, 0, sizeof *cam);
     memset(cfg, 0, sizeof *cfg);
     cam->fov_y_deg = 60.0;
-    cam->width = cam->height = 320;
+    cam->width = cam->height = 640;
     cfg->step = 0.05;
     cfg->max_steps = 3000;
     cfg->samples_per_axis = 1;

@@ -94,7 +94,22 @@ GEON_D void write_result(const geon_djob& J, unsigned long long local, int cls, 
 #pragma unroll
         for (int c = 0; c < 8; ++c) J.yfin[c * J.n_rays + local] = y[c];
     }
-    if (J.done) atomicAdd(J.done, 1ull);
+}
+
+// Adds each lane's v to the counter its p points to, one atomic per distinct
+// counter among the lanes (all lanes of the wave must be here).
+GEON_D void wave_add(unsigned long long* p, uint32_t v) {
+    const unsigned long long mine = (unsigned long long)p;
+    unsigned long long todo = __ballot(v != 0);
+    while (todo) {
+        const int leader = __ffsll((long long)todo) - 1;
+        const unsigned long long lp = __shfl(mine, leader);
+        const bool in = v != 0 && mine == lp;
+        uint32_t sum = in ? v : 0u;
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        if (__lane_id() == (uint32_t)leader) atomicAdd((unsigned long long*)lp, (unsigned long long)sum);
+        todo &= ~__ballot(in);
+    }
 }
 
 // initial state of ray gid in precision T; returns false when the ray is
@@ -245,6 +260,12 @@ __device__ void sched_run(const geon_sched_args& A) {
         }
     };
 
+    // Finished rays are counted per job (the host polls a request's count
+    // after the launch): each lane keeps a pending count for one job and the
+    // wave adds the counts at its refill points and at exit, one atomic per
+    // job there instead of one per ray on the same word.
+    unsigned long long* done_p = nullptr;
+    uint32_t done_n = 0;
     auto finish = [&](int r, int cls) {
         act[r] = false;
         const geon_djob& J = A.jobs[jb[r]];
@@ -256,6 +277,14 @@ __device__ void sched_run(const geon_sched_args& A) {
 #pragma unroll
         for (int c = 0; c < 8; ++c) yd[c] = (double)y[r][c];
         write_result<MP>(J, g[r] - J.ray_base, cls, n[r], yd);
+        if (J.done) {
+            if (J.done != done_p) {
+                if (done_n) atomicAdd(done_p, (unsigned long long)done_n);
+                done_p = J.done;
+                done_n = 0;
+            }
+            ++done_n;
+        }
     };
 
     // resume rays left in flight by the previous quantum
@@ -284,6 +313,8 @@ __device__ void sched_run(const geon_sched_args& A) {
     uint32_t budget = A.quantum;
     for (;;) {
         // ---- refill point (all lanes of the wave converged here) ----
+        wave_add(done_p, done_n);
+        done_n = 0;
         // A wave claims when a quarter of its slots are empty (or all are):
         // one atomicAdd per claim. A claim may run past `end`; positions at or
         // beyond it are simply not used, and between serving quanta the host
@@ -382,6 +413,7 @@ __device__ void sched_run(const geon_sched_args& A) {
         }
         if (A.quantum) budget -= chunk;
     }
+    wave_add(done_p, done_n);
 
     // park rays still in flight
     uint32_t in_flight = 0;
