@@ -420,17 +420,19 @@ namespace {
 
 // Calibration frame of each metric kind: small enough to time in milliseconds,
 // ragged enough (captured, escaped and long rays) to exercise refill, and
-// large enough (410k rays) to fill the persistent grid (256 CUs × 4 SIMDs ×
-// 4 waves × 64 lanes = 262k slots at one ray per lane), so that a shape wins
-// for the load it will carry and not for a half-empty chip.
+// large enough (1M rays, ~4 per slot of the persistent grid's 256 CUs × 4
+// SIMDs × 4 waves × 64 lanes) that shapes are timed at their steady-state
+// throughput. A frame that fits the grid about once favours two rays per
+// lane, which then hold every ray from the start; renders of millions of rays
+// run faster at one (4096² Schwarzschild, FP32-mixed: 296 against 398 ms).
 void calibration(int kind, geon_metric_t* m, geon_camera_t* cam, geon_render_cfg_t* cfg) {
     geon_metric_default(kind, m);
     memset(cam, 0, sizeof *cam);
     memset(cfg, 0, sizeof *cfg);
     cam->fov_y_deg = 60.0;
-    cam->width = cam->height = 640;
+    cam->width = cam->height = 1024;
     cfg->step = 0.05;
-    cfg->max_steps = 3000;
+    cfg->max_steps = 2000;
     cfg->samples_per_axis = 1;
     m->escape_radius = 60.0;
     switch (kind) {
