@@ -22,7 +22,7 @@ from ._abi import (ALCUBIERRE, CAPTURED, CURV_EINSTEIN, CURV_RICCI, CURV_RIEMANN
 __all__ = ["Minkowski", "Schwarzschild", "Alcubierre", "MorrisThorne", "Camera", "RenderConfig",
            "Context", "render", "trace_rays", "riemann", "ricci", "ricci_scalar", "einstein",
            "stress_energy", "nec_contraction", "shadow_bisect", "tetrad", "GeonError", "Batch",
-           "autotune", "Server"]
+           "autotune", "Server", "tonemap_u8", "write_png"]
 
 
 # ---- metrics (src/metric/*.rs) ----------------------------------------------
@@ -448,3 +448,29 @@ def tonemap_u8(rgb):
     out = np.zeros(px.size, np.uint8)
     check(lib().geon_tonemap_u8(_ptr(px), px.size, _ptr(out)), "geon_tonemap_u8")
     return out.reshape(px.shape)
+
+
+def write_png(path, pixels, width: int, height: int):
+    """render::write_png (render.rs:169-184): each linear channel through
+    write_png's tonemap, written as an 8-bit RGB PNG, rows top to bottom. The
+    decoded pixels are the reference's bytes; the compressed stream is
+    zlib's, which the image crate's encoder need not match byte for byte.
+    A buffer whose size is not 3·width·height is refused, as the reference's
+    `from_raw(...).expect` refuses it."""
+    import struct
+    import zlib
+    px = np.asarray(pixels, dtype=np.float32).reshape(-1)
+    if width <= 0 or height <= 0 or px.size != 3 * width * height:
+        raise ValueError("buffer size matches dimensions")
+    rows = tonemap_u8(px).reshape(height, 3 * width)
+    raw = b"".join(b"\x00" + r.tobytes() for r in rows)   # filter type 0 per row
+
+    def chunk(tag, data):
+        return (struct.pack(">I", len(data)) + tag + data +
+                struct.pack(">I", zlib.crc32(tag + data) & 0xFFFFFFFF))
+
+    png = (b"\x89PNG\r\n\x1a\n" +
+           chunk(b"IHDR", struct.pack(">IIBBBBB", width, height, 8, 2, 0, 0, 0)) +
+           chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
+    with open(path, "wb") as f:
+        f.write(png)

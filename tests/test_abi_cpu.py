@@ -88,6 +88,34 @@ def test_tonemap_matches_oracle(O):
     assert np.array_equal(geon.tonemap_u8(px), O.tonemap_u8(px))
 
 
+def test_write_png_pixels(O, tmp_path):
+    """write_png: a valid 8-bit RGB PNG whose decoded rows are the oracle's
+    tonemapped bytes, top row first; a mis-sized buffer is refused."""
+    import struct
+    import zlib
+    import geon_amd as geon
+    rng = np.random.default_rng(2)
+    w, h = 7, 5
+    px = np.concatenate([rng.uniform(-0.2, 1.2, 3 * w * h - 3), [0.0, 1.0, 0.5]]).astype(np.float32)
+    path = tmp_path / "f.png"
+    geon.write_png(path, px, w, h)
+    data = path.read_bytes()
+    assert data[:8] == b"\x89PNG\r\n\x1a\n"
+    chunks, pos = {}, 8
+    while pos < len(data):
+        n = struct.unpack(">I", data[pos:pos + 4])[0]
+        tag, body = data[pos + 4:pos + 8], data[pos + 8:pos + 8 + n]
+        assert struct.unpack(">I", data[pos + 8 + n:pos + 12 + n])[0] == zlib.crc32(tag + body)
+        chunks[tag] = chunks.get(tag, b"") + body
+        pos += 12 + n
+    assert struct.unpack(">IIBBBBB", chunks[b"IHDR"]) == (w, h, 8, 2, 0, 0, 0)
+    raw = np.frombuffer(zlib.decompress(chunks[b"IDAT"]), np.uint8).reshape(h, 1 + 3 * w)
+    assert (raw[:, 0] == 0).all()
+    assert np.array_equal(raw[:, 1:].reshape(-1), O.tonemap_u8(px))
+    with pytest.raises(ValueError):
+        geon.write_png(tmp_path / "g.png", px[:-1], w, h)
+
+
 def test_context_without_gpu_fails_cleanly():
     """On a machine with no visible device the context reports an error code."""
     import torch
