@@ -21,7 +21,7 @@ from ._abi import (ALCUBIERRE, CAPTURED, CURV_EINSTEIN, CURV_RICCI, CURV_RIEMANN
 
 __all__ = ["Minkowski", "Schwarzschild", "Alcubierre", "MorrisThorne", "Camera", "RenderConfig",
            "Context", "render", "trace_rays", "riemann", "ricci", "ricci_scalar", "einstein",
-           "stress_energy", "nec_contraction", "shadow_bisect", "tetrad", "GeonError", "Batch",
+           "stress_energy", "nec_contraction", "nec_scan", "shadow_bisect", "tetrad", "GeonError", "Batch",
            "autotune", "Server", "tonemap_u8", "write_png"]
 
 
@@ -418,15 +418,46 @@ def stress_energy(metric, x, ctx=None):
 
 
 def nec_contraction(metric, x, k, ctx=None):
-    """T_μν k^μ k^ν (curvature.rs:127-136) at each point, summed in the reference order."""
-    T = stress_energy(metric, x, ctx)
+    """T_μν k^μ k^ν (curvature.rs:127-136) at each point, on the device: T
+    from the curvature kernel, the contraction summed in the reference's
+    order. k is one vector or one per point."""
+    ctx = ctx or default_context()
+    xs = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 4).T)
+    n = xs.shape[1]
     k = np.asarray(k, dtype=np.float64).reshape(-1, 4)
-    k = np.broadcast_to(k, (T.shape[0], 4))
-    out = np.zeros(T.shape[0])
-    for mu in range(4):
-        for nu in range(4):
-            out = out + T[:, mu, nu] * k[:, mu] * k[:, nu]
+    out = np.zeros(n)
+    if k.shape[0] == 1:
+        kk = np.ascontiguousarray(k[0])
+        check(lib().geon_nec_contraction(ctx.handle, C.byref(_cmetric(metric)), _ptr(xs), n,
+                                         _ptr(kk), _ptr(out)), "geon_nec_contraction")
+        return out
+    if k.shape[0] != n:
+        raise ValueError("k: one vector or one per point")
+    for row in np.unique(k, axis=0):                # one launch per distinct k
+        sel = np.nonzero((k == row).all(axis=1))[0]
+        xsel = np.ascontiguousarray(xs[:, sel])
+        kk = np.ascontiguousarray(row)
+        part = np.zeros(sel.size)
+        check(lib().geon_nec_contraction(ctx.handle, C.byref(_cmetric(metric)), _ptr(xsel),
+                                         sel.size, _ptr(kk), _ptr(part)), "geon_nec_contraction")
+        out[sel] = part
     return out
+
+
+def nec_scan(metric, x, k, dx=0.0, ctx=None):
+    """The energy-condition line scans of bin/energy_conditions.rs:70-153 over
+    the points x (in order) for one null vector k: returns (nec per point,
+    first strict minimum, its index, Σ nec·dx over the negative values)."""
+    ctx = ctx or default_context()
+    xs = np.ascontiguousarray(np.asarray(x, dtype=np.float64).reshape(-1, 4).T)
+    n = xs.shape[1]
+    kk = np.ascontiguousarray(np.asarray(k, dtype=np.float64).reshape(4))
+    nec = np.zeros(n)
+    mv, mi, integ = C.c_double(), C.c_uint64(), C.c_double()
+    check(lib().geon_nec_scan(ctx.handle, C.byref(_cmetric(metric)), _ptr(xs), n, _ptr(kk),
+                              float(dx), _ptr(nec), C.byref(mv), C.byref(mi), C.byref(integ)),
+          "geon_nec_scan")
+    return nec, mv.value, int(mi.value), integ.value
 
 
 def shadow_bisect(mass, r_obs, step=0.01, max_steps=200_000, iters=60, ctx=None):

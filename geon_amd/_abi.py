@@ -94,6 +94,9 @@ SIGNATURES = [
                              _vp, _vp, _vp, C.POINTER(RenderStats), _vp]),
     ("geon_curvature", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_int32, _vp]),
     ("geon_curvature_dev", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_int32, _vp, _vp]),
+    ("geon_nec_contraction", _i, [_vp, C.POINTER(Metric), _vp, _sz, _vp, _vp]),
+    ("geon_nec_contraction_dev", _i, [_vp, C.POINTER(Metric), _vp, _sz, _vp, _vp, _vp]),
+    ("geon_nec_scan", _i, [_vp, C.POINTER(Metric), _vp, _sz, _vp, C.c_double, _vp, _vp, _vp, _vp]),
     ("geon_shadow_bisect", _i, [_vp, _vp, _vp, _sz, C.c_double, _u32, _u32, _vp, _vp]),
     ("geon_tonemap_u8", _i, [_vp, _sz, _vp]),
     ("geon_batch_create", _i, [_vp, C.POINTER(_vp)]),

@@ -1,7 +1,9 @@
 // geon_abi.cpp — the extern "C" entry points of include/geon_hip.h.
+#include <cmath>
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "geon_runtime.hpp"
@@ -290,6 +292,63 @@ int geon_curvature(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x
     GEON_TRY(geon_curvature_dev(ctx, metric, (const double*)d_x, n, what, (double*)d_out, nullptr));
     GEON_HIP(hipMemcpyDtoHAsync(out, (hipDeviceptr_t)d_out, n * w * 8, s), "D2H");
     GEON_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return GEON_OK;
+}
+
+int geon_nec_contraction_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa,
+                             size_t n, const double k[4], double* out, void* stream) {
+    if (!ctx || !metric || !x_soa || !k || !out) return fail(GEON_E_INVALID, "bad argument");
+    if (n == 0) return GEON_OK;
+    hipSetDevice(ctx->device);
+    void* d_t;
+    GEON_TRY(ctx_buffer(ctx, "nec_t", n * 16 * 8, &d_t));
+    GEON_TRY(geon_curvature_dev(ctx, metric, x_soa, n, GEON_CURV_STRESS, (double*)d_t, stream));
+    hipStream_t s = pick_stream(ctx, stream);
+    uint64_t nn = n;
+    double k0 = k[0], k1 = k[1], k2 = k[2], k3 = k[3];
+    void* args[] = {&d_t, &nn, &k0, &k1, &k2, &k3, &out};
+    return launch(ctx->fn_nec, nn, 256, s, args);
+}
+
+int geon_nec_contraction(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa, size_t n,
+                         const double k[4], double* out) {
+    if (!ctx || !metric || !x_soa || !k || !out) return fail(GEON_E_INVALID, "bad argument");
+    if (n == 0) return GEON_OK;
+    hipSetDevice(ctx->device);
+    void *d_x, *d_out;
+    GEON_TRY(ctx_buffer(ctx, "nec_x", n * 32, &d_x));
+    GEON_TRY(ctx_buffer(ctx, "nec_out", n * 8, &d_out));
+    hipStream_t s = ctx->stream;
+    GEON_HIP(hipMemcpyHtoDAsync((hipDeviceptr_t)d_x, (void*)x_soa, n * 32, s), "H2D");
+    GEON_TRY(geon_nec_contraction_dev(ctx, metric, (const double*)d_x, n, k, (double*)d_out, nullptr));
+    GEON_HIP(hipMemcpyDtoHAsync(out, (hipDeviceptr_t)d_out, n * 8, s), "D2H");
+    GEON_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+    return GEON_OK;
+}
+
+// The energy-condition line scans (energy_conditions.rs:70-153): the
+// contraction at each point, then, in point order, the first strict
+// minimum and the sum of nec·dx over the negative values.
+int geon_nec_scan(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa, size_t n,
+                  const double k[4], double dx, double* nec_out, double* min_value,
+                  uint64_t* min_index, double* neg_integral) {
+    if (!ctx || !metric || !x_soa || !k || !min_value || !min_index || !neg_integral)
+        return fail(GEON_E_INVALID, "bad argument");
+    std::vector<double> v(n);
+    GEON_TRY(geon_nec_contraction(ctx, metric, x_soa, n, k, v.data()));
+    double mv = INFINITY, integral = 0.0;
+    uint64_t mi = 0;
+    for (size_t i = 0; i < n; ++i) {
+        if (v[i] < mv) {
+            mv = v[i];
+            mi = i;
+        }
+        if (v[i] < 0.0) integral += v[i] * dx;
+    }
+    if (nec_out) std::copy(v.begin(), v.end(), nec_out);
+    *min_value = mv;
+    *min_index = mi;
+    *neg_integral = integral;
     return GEON_OK;
 }
 

@@ -202,6 +202,23 @@ extern "C" __global__ __launch_bounds__(256) void geon_job_finalize(const geon_d
                     J->stats);
 }
 
+// T_μν k^μ k^ν at each of n points from their T (n × 16, row-major per
+// point), summed over μ then ν from 0.0 as (T_μν·k^μ)·k^ν
+// (curvature.rs:127-136).
+extern "C" __global__ void geon_nec_kk(const double* __restrict__ T, uint64_t n, double k0, double k1,
+                                       double k2, double k3, double* __restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double k[4] = {k0, k1, k2, k3};
+    const double* t = T + i * 16;
+    double s = 0.0;
+#pragma unroll
+    for (int mu = 0; mu < 4; ++mu)
+#pragma unroll
+        for (int nu = 0; nu < 4; ++nu) s += t[mu * 4 + nu] * k[mu] * k[nu];
+    out[i] = s;
+}
+
 // Zero n 64-bit words (counters, cursors, statistics) without a memset node.
 extern "C" __global__ void geon_zero_u64(unsigned long long* p, uint64_t n) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;

@@ -17,6 +17,9 @@
  *   geon_render[_dev]        render::render            (src/render.rs:55-165)
  *   geon_curvature[_dev]     curvature::{riemann,ricci,ricci_scalar,einstein,
  *                            stress_energy}            (src/curvature.rs:48-126)
+ *   geon_nec_contraction[_dev] curvature::nec_contraction (src/curvature.rs:127-136)
+ *   geon_nec_scan            the line scans of bin/energy_conditions.rs:70-153
+ *                            (strict minimum, ∫ of the negative part)
  *   geon_shadow_bisect       bin/validate_shadow.rs:20-95 (is_captured +
  *                            bisection on the launch angle)
  *   geon_tonemap_u8          render::write_png tonemap (src/render.rs:169-180)
@@ -193,6 +196,22 @@ int  geon_curvature(geon_ctx_t* ctx, const geon_metric_t* metric, const double* 
                     size_t n, int32_t what, double* out);
 int  geon_curvature_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa,
                         size_t n, int32_t what, double* out, void* stream);
+
+/* T_μν k^μ k^ν at n points x (SoA 4×n) for one vector k, summed over μ then
+ * ν as in curvature::nec_contraction (src/curvature.rs:127-136); T_μν is
+ * GEON_CURV_STRESS. out: n doubles. */
+int  geon_nec_contraction(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa,
+                          size_t n, const double k[4], double* out);
+int  geon_nec_contraction_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa,
+                              size_t n, const double k[4], double* out, void* stream);
+/* The energy-condition line scans (src/bin/energy_conditions.rs:70-153): the
+ * contraction at the n points (nec_out, optional), the first strict minimum
+ * in point order (min_value, min_index; +inf and 0 for n = 0) and Σ nec·dx
+ * over the points where nec < 0, accumulated in point order (the ∫ T_kk dx
+ * proxy of :105-118). */
+int  geon_nec_scan(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa, size_t n,
+                   const double k[4], double dx, double* nec_out, double* min_value,
+                   uint64_t* min_index, double* neg_integral);
 
 /*
  * Shadow bisection (validate_shadow.rs): for each of n independent

@@ -207,6 +207,45 @@ def test_energy_condition_anchors_on_gpu(ctx, anchors):
     assert float(f"{T[0, 0]:.3g}") == float(f"{-1 / (8 * math.pi):.3g}")
 
 
+def test_energy_condition_scans_on_gpu(ctx, O, anchors):
+    """The line scans of bin/energy_conditions.rs on the device (geon_nec_scan):
+    the Alcubierre minimum over x = −4 + 0.2n and the ∫ of the negative part
+    over x = −5, −5 + 0.05, … ≤ 5 (accumulated as the harness does), and the
+    Morris–Thorne minimum at the throat, against docs/energy_conditions.txt
+    and against the oracle point by point. The Alcubierre minimum has a mirror
+    point at −2.40 whose value differs from +2.40's only by FD noise, so the
+    scan may keep either."""
+    import geon_amd as geon
+    rows = {r["kind"]: r for r in anchors["energy_conditions"]["rows"]}
+    k = [1.0, 1.0, 0.0, 0.0]
+    alc = geon.Alcubierre(bubble_radius=2.0, bubble_thickness=0.5, ship_velocity=2.0)
+    oalc = O.metric(O.ALCUBIERRE, bubble_radius=2.0, bubble_thickness=0.5, ship_velocity=2.0,
+                    ship_x=0.0)
+    xs = np.array([[0.0, -4.0 + 0.2 * n, 0.0, 0.0] for n in range(41)])
+    nec, mv, mi, _ = geon.nec_scan(alc, xs, k, ctx=ctx)
+    ref = np.array([O.nec_contraction(oalc, x, k) for x in xs])
+    print(f"alcubierre scan: min {mv:.6e} at x = {xs[mi, 1]:.2f}; max |Δ| vs oracle "
+          f"{np.abs(nec - ref).max():.3e}", flush=True)
+    assert np.abs(nec - ref).max() <= 1e-5 * np.abs(ref).max()
+    want = rows["alcubierre_nec_min"]
+    assert abs(mv - want["value"]) <= 1e-4 * abs(want["value"])
+    assert f"{abs(xs[mi, 1]):.2f}" == "2.40" and nec[mi] == nec.min()
+    pts, x_pos = [], -5.0
+    while x_pos <= 5.0:
+        pts.append([0.0, x_pos, 0.0, 0.0])
+        x_pos += 0.05
+    _, _, _, integral = geon.nec_scan(alc, pts, k, dx=0.05, ctx=ctx)
+    want = rows["alcubierre_nec_integral"]["value"]
+    print(f"alcubierre ∫ negative part: {integral:.6e} (oracle {O.alcubierre_nec_integral(oalc):.6e})")
+    assert abs(integral - want) <= 1e-4 * abs(want)
+    assert abs(integral - O.alcubierre_nec_integral(oalc)) <= 1e-6 * abs(want)
+    mt = geon.MorrisThorne(throat_radius=1.0)
+    xs = np.array([[0.0, -4.0 + 0.2 * n, math.pi / 2, 0.0] for n in range(41)])
+    nec, mv, mi, _ = geon.nec_scan(mt, xs, k, ctx=ctx)
+    want = rows["morris_thorne_nec_min"]
+    assert abs(mv - want["value"]) <= 1e-4 * abs(want["value"]) and mi == 20
+
+
 def test_shadow_bisection_readme(ctx, O, anchors):
     """validate_shadow: 60 halvings, step 0.01, max 200,000 (README.md:43-45)."""
     import geon_amd as geon
