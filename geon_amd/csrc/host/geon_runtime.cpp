@@ -145,6 +145,14 @@ int make_dmetric(const geon_metric_t* m, geon_dmetric* d) {
         // FD step of margin every stencil point of a ray beyond this radius
         // sees f == 0.
         d->flat_rs = d->R + 22.0 * fabs(d->s) + 2.0 * m->fd_step;
+        d->inv_denom = 1.0 / d->denom;
+        // Quotients by 2h and by denom are formed from the correctly rounded
+        // reciprocals (one multiply and two FMAs) where every operand and
+        // quotient is a normal number far from overflow: f ≤ 2/denom, the
+        // metric entries are bounded by 1 + v²f², and their differences are
+        // 0 or at least an ulp of 1.
+        d->mk = (d->s > 0.0 && d->denom >= 1e-3 && d->denom <= 2.0 && fabs(d->v) <= 1e6 &&
+                 d->fd_two_h >= 1e-150 && d->fd_two_h <= 1e150) ? 1 : 0;
     }
     if (m->kind == GEON_METRIC_MORRIS_THORNE) {
         d->b0 = m->p[0];

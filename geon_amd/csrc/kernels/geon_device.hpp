@@ -276,6 +276,9 @@ struct FD {
                 if constexpr (MP::dg_nz(a, mu, nu)) {
                     if constexpr (nu < mu && MP::dg_nz(a, nu, mu))
                         dg[a * 16 + mu * 4 + nu] = dg[a * 16 + nu * 4 + mu];  // symmetric g
+                    else if (P.mk)   // a difference of two metric values is never −0
+                        dg[a * 16 + mu * 4 + nu] =
+                            div_rn_pz(gp[mu * 4 + nu] - gm[mu * 4 + nu], P.fd_two_h, P.fd_inv_2h);
                     else
                         dg[a * 16 + mu * 4 + nu] = (gp[mu * 4 + nu] - gm[mu * 4 + nu]) / P.fd_two_h;
                 }
@@ -637,7 +640,8 @@ struct Alcubierre {
             tp = shape_tanh<INLINE_TANH>((r_s + P.R) / P.s);
             tm = shape_tanh<INLINE_TANH>((r_s - P.R) / P.s);
         }
-        return (tp - tm) / P.denom;
+        // tp ≥ tm, so the difference is never −0
+        return P.mk ? div_rn_pz(tp - tm, P.denom, P.inv_denom) : (tp - tm) / P.denom;
     }
     // g from the shape value f at the point (alcubierre.rs:48-63)
     GEON_D static void g_of_f(const geon_dmetric& P, double f, double* g) {
@@ -651,14 +655,23 @@ struct Alcubierre {
         g_of_f(P, shape(P, rs(P, x)), g);
     }
     // Cofactor inverse of [[a,b],[b,1]] ⊕ I: det = a − b², g^00 = 1/det,
-    // g^01 = g^10 = −b/det, g^11 = a/det, g^22 = g^33 = det/det = 1.
-    GEON_D static void ginv(const geon_dmetric&, const double* g, double* gi) {
+    // g^01 = g^10 = −b/det, g^11 = a/det, g^22 = g^33 = det/det = 1. The
+    // reciprocal serves the other two quotients (a = −1 + v²f² is never −0).
+    GEON_D static void ginv(const geon_dmetric& P, const double* g, double* gi) {
         const double a = g[0], b = g[1];
         const double det = a - b * b;
-        gi[0] = 1.0 / det;
-        gi[1] = -b / det;
+        const double ad = fabs(det);
+        if (P.mk && ad > 0x1p-500 && ad < 0x1p500) {
+            const double y = 1.0 / det;
+            gi[0] = y;
+            gi[1] = div_rn(-b, det, y);
+            gi[5] = div_rn_pz(a, det, y);
+        } else {
+            gi[0] = 1.0 / det;
+            gi[1] = -b / det;
+            gi[5] = a / det;
+        }
         gi[4] = gi[1];
-        gi[5] = a / det;
         gi[10] = 1.0;
         gi[15] = 1.0;
     }
