@@ -146,6 +146,10 @@ GEON_D void sincos_cw(double x, double* s, double* c) {
 // NaN stays NaN; the sign of x is copied (tanh(−0) = −0).
 GEON_D double tanh_cw(double x) {
     const double ax = fabs(x);
+    // |x| ≥ 19.1: 1 − tanh|x| = 2/(e^{2|x|} + 1) < 2^-54, half the spacing
+    // below 1, so tanh x rounds to ±1 (the wall arguments (r_s + R)/s of an
+    // Alcubierre bubble are past this almost everywhere)
+    if (ax >= 19.1) return __builtin_copysign(1.0, x);
     const double a = ax > 22.0 ? 22.0 : ax;
     const double t = a + a;
     const double k = rint(t * 0x1.71547652b82fep+0);            // t / ln 2
