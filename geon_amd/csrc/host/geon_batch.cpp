@@ -420,19 +420,21 @@ namespace {
 
 // Calibration frame of each metric kind: small enough to time in milliseconds,
 // ragged enough (captured, escaped and long rays) to exercise refill, and
-// large enough (1M rays, ~4 per slot of the persistent grid's 256 CUs × 4
+// large enough (4M rays, ~16 per slot of the persistent grid's 256 CUs × 4
 // SIMDs × 4 waves × 64 lanes) that shapes are timed at their steady-state
-// throughput. A frame that fits the grid about once favours two rays per
-// lane, which then hold every ray from the start; renders of millions of rays
-// run faster at one (4096² Schwarzschild, FP32-mixed: 296 against 398 ms).
+// throughput. On frames that fill the grid only a few times the fill and the
+// tail weigh enough to favour two rays per lane, which hold every ray from
+// the start; renders of millions of rays run faster at one (4096²
+// Schwarzschild, FP32-mixed: 296 against 398 ms). 1000 steps per ray keep a
+// candidate's timing near 0.1 s in FP64.
 void calibration(int kind, geon_metric_t* m, geon_camera_t* cam, geon_render_cfg_t* cfg) {
     geon_metric_default(kind, m);
     memset(cam, 0, sizeof *cam);
     memset(cfg, 0, sizeof *cfg);
     cam->fov_y_deg = 60.0;
-    cam->width = cam->height = 1024;
+    cam->width = cam->height = 2048;
     cfg->step = 0.05;
-    cfg->max_steps = 2000;
+    cfg->max_steps = 1000;
     cfg->samples_per_axis = 1;
     m->escape_radius = 60.0;
     switch (kind) {
