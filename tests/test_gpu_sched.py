@@ -157,6 +157,33 @@ def test_mixed_precision_generic_metrics_vs_oracle(ctx, O):
         assert ang.mean() < 1e-4 and ang.max() < 1e-3
 
 
+def test_mixed_precision_morris_thorne_headline_frame(ctx):
+    """FP32-mixed against the FP64 scheduler on the Morris–Thorne 1024² frame
+    (b0 = 1, l = 15, 20,000 steps): no class mismatch and every escaped
+    direction within 1e-3 rad, leaving out the pole-flung rays whose
+    direction is (0, 0, 0) in both precisions (their states overflow the
+    direction formula)."""
+    import geon_amd as geon
+    m = geon.MorrisThorne(throat_radius=1.0)
+    cam = geon.Camera((0.0, 15.0, math.pi / 2, 0.0), 60.0, 1024, 1024)
+    out = {}
+    for prec in (geon.PREC_FP64, geon.PREC_MIXED):
+        (res,), _, tot = _run_frames(ctx, [(m, cam, geon.RenderConfig(step=0.05, max_steps=20000,
+                                                                        precision=prec))])
+        out[prec] = (res, tot)
+    (_, c64, d64), _ = out[geon.PREC_FP64]
+    (_, cmx, dmx), tot = out[geon.PREC_MIXED]
+    assert np.array_equal(c64, cmx)
+    n64 = np.linalg.norm(d64, axis=1)
+    flung = (c64 == 2) & (n64 == 0.0)
+    assert np.array_equal(dmx[flung], d64[flung])
+    esc = (c64 == 2) & ~flung
+    ang = _angle(dmx[esc], d64[esc])
+    print(f"MT 1024² mixed: escaped {int(esc.sum())}, flung {int(flung.sum())}, re-traced in FP64 "
+          f"{tot['fp64_rays']}, angle max {ang.max():.3e}", flush=True)
+    assert esc.sum() > 3000 and ang.max() < 1e-3
+
+
 def test_explicit_rays_early_exit_matches_full_trace(ctx, O):
     """Capture predicate with the photon-sphere early exit (bisection rays),
     Schwarzschild M=1, r_obs=20, step 0.01, max 200,000."""
