@@ -141,10 +141,11 @@ int make_dmetric(const geon_metric_t* m, geon_dmetric* d) {
         d->s_pow2 = is_pow2(d->s) ? 1 : 0;
         d->inv_s = 1.0 / d->s;
         d->denom = 2.0 * tanh(d->R / d->s);           // alcubierre.rs:42
-        // tanh(u) is exactly 1.0 for u >= 22 in IEEE double; with one more
-        // FD step of margin every stencil point of a ray beyond this radius
-        // sees f == 0.
-        d->flat_rs = d->R + 22.0 * fabs(d->s) + 2.0 * m->fd_step;
+        // tanh(u) rounds to exactly 1.0 for u ≥ 19.07 (1 − tanh u < 2^-54):
+        // glibc's tanh and the kernels' return 1.0 from 19.1 on. With one
+        // more FD step of margin (h/s ≫ the rounding of u) every stencil
+        // point of a ray beyond this radius sees f == 0.
+        d->flat_rs = d->R + 19.1 * fabs(d->s) + 2.0 * m->fd_step;
         d->inv_denom = 1.0 / d->denom;
         // Quotients by 2h and by denom are formed from the correctly rounded
         // reciprocals (one multiply and two FMAs) where every operand and
