@@ -194,8 +194,11 @@ int batch_run_on(geon_batch* b, const geon_sched_cfg_t* tune, hipStream_t user) 
             a.fb_cap = b->fb_cap[k];
             a.ray_steps = ck + C_STEPS32;
             GEON_TRY(sched_launch(c, k, 1, cfg, a, b->q_rays[k][1], s));
-            // FP64 pass over the rays the FP32 pass handed back
+            // FP64 pass over the rays the FP32 pass handed back, one ray per
+            // lane as in serving: the list is short and its rays are the long
+            // ones (near capture, pole passes), so the pass ends on its tail
             geon_sched_cfg_t cf = resolve_cfg(c, k, 0, nullptr);
+            cf.rays_per_lane = 1;
             geon_sched_args f;
             memset(&f, 0, sizeof f);
             f.jobs = b->d_jobs[k][1];
