@@ -194,11 +194,15 @@ int batch_run_on(geon_batch* b, const geon_sched_cfg_t* tune, hipStream_t user) 
             a.fb_cap = b->fb_cap[k];
             a.ray_steps = ck + C_STEPS32;
             GEON_TRY(sched_launch(c, k, 1, cfg, a, b->q_rays[k][1], s));
-            // FP64 pass over the rays the FP32 pass handed back, one ray per
-            // lane as in serving: the list is short and its rays are the long
-            // ones (near capture, pole passes), so the pass ends on its tail
+            // FP64 pass over the rays the FP32 pass handed back. Its length is
+            // known only on the device; it runs near a tenth of the FP32 queue.
+            // A short list (4096² Schwarzschild: 1.37 M rays, ~5 per slot of
+            // the one-ray grid) ends on its tail and runs faster at one ray per
+            // lane (286 against 297 ms for the frame); long lists (the ragged
+            // batch's 3–4 M per kind) keep the tuned shape, two rays per lane
+            // in flight (1.47–1.50 against 1.54 s for the batch).
             geon_sched_cfg_t cf = resolve_cfg(c, k, 0, nullptr);
-            cf.rays_per_lane = 1;
+            if (b->q_rays[k][1] < 24000000ull) cf.rays_per_lane = 1;
             geon_sched_args f;
             memset(&f, 0, sizeof f);
             f.jobs = b->d_jobs[k][1];
