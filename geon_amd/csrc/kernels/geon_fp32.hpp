@@ -59,7 +59,10 @@ GEON_D FParams fparams(const geon_dmetric& P) {
     // 1 − 2/(e^{2u} + 1) rounds to 1.0f for u ≥ 9.1; 10 wall widths of margin
     f.flat_rs = (float)(P.R + 10.0 * fabs(P.s));
     f.near_orbit = P.kind == 1 ? (float)(3.2 * P.M) : (P.kind == 3 ? (float)(0.1 * P.b0) : 0.0f);
-    f.pole_r = P.kind == 1 ? (float)(6.0 * P.M) : 1e30f;
+    // the wide band near the hole or the throat only (Morris–Thorne: 62.7 M
+    // rays of 89 frames against FP64, the same worst escape direction, 5.0e-4
+    // rad, with it at every radius or within 6 b0; without it, 3 rad)
+    f.pole_r = P.kind == 1 ? (float)(6.0 * P.M) : (P.kind == 3 ? (float)(6.0 * P.b0) : 1e30f);
     return f;
 }
 
