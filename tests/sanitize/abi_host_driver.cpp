@@ -51,6 +51,15 @@ int main() {
     bad += expect(geon_batch_create(nullptr, &b), GEON_E_INVALID, "batch without context");
     geon_server_t* srv = nullptr;
     bad += expect(geon_serve_create(nullptr, nullptr, &srv), GEON_E_INVALID, "server without context");
+    geon_metric_t mt;
+    geon_metric_default(GEON_METRIC_MORRIS_THORNE, &mt);
+    const double xs[4] = {0.0, 0.0, 1.5707963267948966, 0.0}, kv[4] = {1.0, 1.0, 0.0, 0.0};
+    double nec = 0.0, vmin = 0.0, integral = 0.0;
+    uint64_t imin = 0;
+    bad += expect(geon_nec_contraction(nullptr, &mt, xs, 1, kv, &nec), GEON_E_INVALID,
+                  "nec contraction without context");
+    bad += expect(geon_nec_scan(nullptr, &mt, xs, 1, kv, 0.05, nullptr, &vmin, &imin, &integral),
+                  GEON_E_INVALID, "nec scan without context");
     printf("driver done, failures %d\n", bad);
     return bad ? 1 : 0;
 }
