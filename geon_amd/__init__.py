@@ -20,7 +20,7 @@ from ._abi import (ALCUBIERRE, CAPTURED, CURV_EINSTEIN, CURV_RICCI, CURV_RIEMANN
                    PREC_FP64, PREC_MIXED, SCHWARZSCHILD, GeonError, check, lib)
 
 __all__ = ["Minkowski", "Schwarzschild", "Alcubierre", "MorrisThorne", "Camera", "RenderConfig",
-           "Context", "render", "trace_rays", "riemann", "ricci", "ricci_scalar", "einstein",
+           "Context", "render", "trace_rays", "trace_rays_adaptive", "riemann", "ricci", "ricci_scalar", "einstein",
            "stress_energy", "nec_contraction", "nec_scan", "shadow_bisect", "tetrad", "GeonError", "Batch",
            "autotune", "Server", "tonemap_u8", "write_png"]
 
@@ -206,6 +206,27 @@ def trace_rays(metric, y0, step, max_steps, ctx: Context | None = None):
                                 int(max_steps), _ptr(cls), _ptr(steps), _ptr(yf), _ptr(resid),
                                 _ptr(d)), "geon_trace_rays")
     return dict(cls=cls, steps=steps, y=yf.T.copy(), resid=resid, dir=d.T.copy())
+
+
+def trace_rays_adaptive(metric, y0, h0, rtol, atol, max_steps, ctx: Context | None = None):
+    """trace_rays with adaptive Cash–Karp RK4(5) steps (the adaptive stepping
+    integrator.rs:5-7 names as the next step; not a reference method).
+
+    Returns dict(cls, steps, rejected, y, dir) with y n×8 and dir n×3.
+    """
+    ctx = ctx or default_context()
+    y0 = np.ascontiguousarray(np.asarray(y0, dtype=np.float64).reshape(-1, 8).T)  # SoA
+    n = y0.shape[1]
+    cls = np.zeros(n, np.int32)
+    steps = np.zeros(n, np.uint32)
+    rej = np.zeros(n, np.uint32)
+    yf = np.zeros((8, n))
+    d = np.zeros((3, n))
+    check(lib().geon_trace_rays_adaptive(ctx.handle, C.byref(_cmetric(metric)), _ptr(y0), n,
+                                         float(h0), float(rtol), float(atol), int(max_steps),
+                                         _ptr(cls), _ptr(steps), _ptr(rej), _ptr(yf), _ptr(d)),
+          "geon_trace_rays_adaptive")
+    return dict(cls=cls, steps=steps, rejected=rej, y=yf.T.copy(), dir=d.T.copy())
 
 
 def render(metric, camera: Camera, cfg: RenderConfig | None = None, ctx: Context | None = None):

@@ -75,9 +75,10 @@ def build_opcount(out=None):
     out = out or os.path.join(LIB, "opcount")
     os.makedirs(os.path.dirname(out), exist_ok=True)
     cxx = shutil.which("g++") or "g++"
+    tmp = f"{out}.{os.getpid()}.tmp"   # concurrent test workers each build their own
     _run([cxx, "-O1", "-std=c++17", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
-          OPCOUNT_SRC, "-o", out + ".tmp"])
-    os.replace(out + ".tmp", out)
+          OPCOUNT_SRC, "-o", tmp])
+    os.replace(tmp, out)
     return out
 
 

@@ -204,6 +204,7 @@ int ctx_create(int device, const char* code_object, geon_ctx** out) {
     for (int k = 0; k < 4 && ok; ++k) {
         ok = ok && get(&c->fn_render[k], std::string("geon_render_rays_") + kMetricNames[k]);
         ok = ok && get(&c->fn_trace[k], std::string("geon_trace_states_") + kMetricNames[k]);
+        ok = ok && get(&c->fn_trace_adaptive[k], std::string("geon_trace_adaptive_") + kMetricNames[k]);
         ok = ok && get(&c->fn_curv[k], std::string("geon_curvature_") + kMetricNames[k]);
     }
     ok = ok && get(&c->fn_finalize, "geon_render_finalize");

@@ -13,6 +13,7 @@ struct geon_ctx {
     hipModule_t mod = nullptr;
     hipFunction_t fn_render[4] = {};
     hipFunction_t fn_trace[4] = {};
+    hipFunction_t fn_trace_adaptive[4] = {};
     hipFunction_t fn_curv[4] = {};
     hipFunction_t fn_finalize = nullptr;
     hipFunction_t fn_stats_clear = nullptr;

@@ -1018,6 +1018,119 @@ GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps,
     return MAXSTEPS;
 }
 
+// ---- adaptive stepping -------------------------------------------------------
+// The reference integrates with fixed steps and names adaptive stepping as its
+// next step (integrator.rs:5-7); this is that step, not a reference method.
+// Cash–Karp embedded RK4(5) (J. R. Cash, A. H. Karp, ACM TOMS 16 (1990)
+// 201-222): six derivative evaluations, the fifth-order solution advances and
+// its difference to the fourth-order one estimates the local error. Every sum
+// runs left to right in the tableau's order. (Not part of the op-counting
+// build, whose scalar stands in for double.)
+#ifndef GEON_OPCOUNT
+namespace ck {
+constexpr double a21 = 1.0 / 5.0;
+constexpr double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;
+constexpr double a41 = 3.0 / 10.0, a42 = -9.0 / 10.0, a43 = 6.0 / 5.0;
+constexpr double a51 = -11.0 / 54.0, a52 = 5.0 / 2.0, a53 = -70.0 / 27.0, a54 = 35.0 / 27.0;
+constexpr double a61 = 1631.0 / 55296.0, a62 = 175.0 / 512.0, a63 = 575.0 / 13824.0,
+                 a64 = 44275.0 / 110592.0, a65 = 253.0 / 4096.0;
+constexpr double b1 = 37.0 / 378.0, b3 = 250.0 / 621.0, b4 = 125.0 / 594.0, b6 = 512.0 / 1771.0;
+// fifth- minus fourth-order weights
+constexpr double e1 = b1 - 2825.0 / 27648.0, e3 = b3 - 18575.0 / 48384.0,
+                 e4 = b4 - 13525.0 / 55296.0, e5 = -277.0 / 14336.0, e6 = b6 - 1.0 / 4.0;
+}  // namespace ck
+
+// dy/dλ = (p, dp/dλ) (geodesic.rs:38-59)
+template <class MP>
+GEON_D void geodesic_rhs(const geon_dmetric& P, const double* y, double* k) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) k[i] = y[4 + i];
+    MP::accel(P, y, k + 4);
+}
+
+template <class MP>
+GEON_D void ck_step(const geon_dmetric& P, const double* y, double h, double* yn, double* err) {
+    using namespace ck;
+    double k1[8], k2[8], k3[8], k4[8], k5[8], k6[8], yt[8];
+    geodesic_rhs<MP>(P, y, k1);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) yt[i] = y[i] + h * (a21 * k1[i]);
+    geodesic_rhs<MP>(P, yt, k2);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) yt[i] = y[i] + h * (a31 * k1[i] + a32 * k2[i]);
+    geodesic_rhs<MP>(P, yt, k3);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) yt[i] = y[i] + h * (a41 * k1[i] + a42 * k2[i] + a43 * k3[i]);
+    geodesic_rhs<MP>(P, yt, k4);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        yt[i] = y[i] + h * (a51 * k1[i] + a52 * k2[i] + a53 * k3[i] + a54 * k4[i]);
+    geodesic_rhs<MP>(P, yt, k5);
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        yt[i] = y[i] + h * (a61 * k1[i] + a62 * k2[i] + a63 * k3[i] + a64 * k4[i] + a65 * k5[i]);
+    geodesic_rhs<MP>(P, yt, k6);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        yn[i] = y[i] + h * (b1 * k1[i] + b3 * k3[i] + b4 * k4[i] + b6 * k6[i]);
+        err[i] = h * (e1 * k1[i] + e3 * k3[i] + e4 * k4[i] + e5 * k5[i] + e6 * k6[i]);
+    }
+}
+
+// trace with adaptive steps: the termination test before each step, the NaN
+// guard after it and max_steps (counting accepted steps) as in trace. A step
+// is accepted when e = max_i |err_i| / (atol + rtol·max(|y_i|, |y_new_i|)) ≤ 1;
+// the next step is h·0.9·e^(−1/4), at most 5h (e below 1.0498e-3) and at
+// least h/10, and |h| stays within [1e-8, 8]·|h0| (the upper bound keeps a
+// step from jumping a feature the stages would miss) — square roots and
+// divisions only, which host and device round alike. A step falling below
+// the lower bound ends the trace as DIVERGED.
+template <class MP>
+GEON_D int trace_adaptive(const geon_dmetric& P, double* y, double h0, double rtol, double atol,
+                          uint32_t max_steps, uint32_t* steps, uint32_t* rejected) {
+    double h = h0;
+    const double h_min = fabs(h0) * 1e-8, h_max = fabs(h0) * 8.0;
+    uint32_t n = 0, rej = 0;
+    int cls = MAXSTEPS;
+    while (n < max_steps) {
+        const int t = MP::terminate(P, y);
+        if (t != RUNNING) {
+            cls = t;
+            break;
+        }
+        double yn[8], err[8];
+        ck_step<MP>(P, y, h, yn, err);
+        double e = 0.0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double sc = atol + rtol * fmax(fabs(y[i]), fabs(yn[i]));
+            e = fmax(e, fabs(err[i]) / sc);
+        }
+        if (e > 1.0) {
+            ++rej;
+            h = h * fmax(0.9 / sqrt(sqrt(e)), 0.1);
+            if (!(fabs(h) >= h_min)) {
+                cls = DIVERGED;
+                break;
+            }
+            continue;
+        }
+        if (!finite8(yn)) {
+            cls = DIVERGED;
+            break;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) y[i] = yn[i];
+        ++n;
+        h = h * (e > 1.0498e-3 ? 0.9 / sqrt(sqrt(e)) : 5.0);
+        if (fabs(h) > h_max) h = copysign(h_max, h);
+    }
+    *steps = n;
+    *rejected = rej;
+    return cls;
+}
+#endif  // GEON_OPCOUNT
+
 // camera.rs:113-160 — pixel (i, j) + (du, dv) → initial state.
 GEON_D void camera_ray(const geon_dframe& F, uint32_t i, uint32_t j, double du, double dv,
                        double* y) {

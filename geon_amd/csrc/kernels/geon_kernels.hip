@@ -76,6 +76,37 @@ __device__ void trace_states(uint32_t slot, uint64_t n, const double* __restrict
     }
 }
 
+// trace_states with adaptive steps (trace_adaptive): h0 = the frame's step,
+// max_steps counts accepted steps; `rejected` receives the rejected attempts.
+template <class MP>
+__device__ void trace_states_adaptive(uint32_t slot, uint64_t n, const double* __restrict__ y0,
+                                      double rtol, double atol, int32_t* __restrict__ cls,
+                                      uint32_t* __restrict__ steps, uint32_t* __restrict__ rejected,
+                                      double* __restrict__ yf, double* __restrict__ dir) {
+    const geon_dframe& F = c_frames[slot];
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    double y[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) y[c] = y0[c * n + k];
+    uint32_t st, rj;
+    const int c = trace_adaptive<MP>(F.m, y, F.step, rtol, atol, F.max_steps, &st, &rj);
+    if (cls) cls[k] = c;
+    if (steps) steps[k] = st;
+    if (rejected) rejected[k] = rj;
+    if (yf) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) yf[q * n + k] = y[q];
+    }
+    if (dir) {
+        double d[3] = {0.0, 0.0, 0.0};
+        if (c == ESCAPED) direction<MP>(y, d);
+        dir[k] = d[0];
+        dir[n + k] = d[1];
+        dir[2 * n + k] = d[2];
+    }
+}
+
 // WAVES: the minimum waves per SIMD the register allocation must allow.
 // Alcubierre's full step (seven metric evaluations per stage) otherwise takes
 // over 256 VGPRs, one wave per SIMD; at four it spills four.
@@ -89,6 +120,11 @@ __device__ void trace_states(uint32_t slot, uint64_t n, const double* __restrict
         uint32_t slot, uint64_t n, const double* y0, int32_t* cls, uint32_t* steps,            \
         double* yf, double* resid, double* dir) {                                               \
         trace_states<MP>(slot, n, y0, cls, steps, yf, resid, dir);                              \
+    }                                                                                           \
+    extern "C" __global__ __launch_bounds__(256, WAVES) void geon_trace_adaptive_##NAME(        \
+        uint32_t slot, uint64_t n, const double* y0, double rtol, double atol, int32_t* cls,   \
+        uint32_t* steps, uint32_t* rejected, double* yf, double* dir) {                         \
+        trace_states_adaptive<MP>(slot, n, y0, rtol, atol, cls, steps, rejected, yf, dir);       \
     }
 
 GEON_KERNELS(mink, Minkowski, 1)

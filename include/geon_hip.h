@@ -14,6 +14,9 @@
  *                             src/metric/minkowski.rs:8)
  *   geon_tetrad              Camera::tetrad            (src/camera.rs:47-97)
  *   geon_trace_rays[_dev]    geodesic::trace, batched  (src/geodesic.rs:62-117)
+ *   geon_trace_rays_adaptive no counterpart: the adaptive stepping the
+ *                            reference names as its next step
+ *                            (src/integrator.rs:5-7), same trace semantics
  *   geon_render[_dev]        render::render            (src/render.rs:55-165)
  *   geon_curvature[_dev]     curvature::{riemann,ricci,ricci_scalar,einstein,
  *                            stress_energy}            (src/curvature.rs:48-126)
@@ -190,6 +193,16 @@ int  geon_render(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_camera
 int  geon_render_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_camera_t* cam,
                      const geon_render_cfg_t* cfg, float* rgb, int32_t* cls_per_pixel,
                      double* dir_per_pixel, geon_render_stats_t* stats, void* stream);
+
+/* geon_trace_rays with adaptive steps: Cash–Karp RK4(5), a step accepted
+ * when max_i |err_i| / (atol + rtol·max(|y_i|, |y_new_i|)) ≤ 1, next step
+ * h·0.9·e^(−1/4) in [h/10, 5h], |h| within [1e-8, 8]·|h0| (below: DIVERGED).
+ * max_steps counts accepted steps; rejected (optional) the rejected attempts.
+ * h0 ≠ 0, rtol ≥ 0, atol > 0. Outputs as geon_trace_rays (no residual). */
+int  geon_trace_rays_adaptive(geon_ctx_t* ctx, const geon_metric_t* metric, const double* y0_soa,
+                              size_t n, double h0, double rtol, double atol, uint32_t max_steps,
+                              int32_t* cls, uint32_t* steps, uint32_t* rejected,
+                              double* y_final_soa, double* dir_soa);
 
 /* Curvature at n points x (SoA 4×n). out: n×{256|16|1} doubles. */
 int  geon_curvature(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa,

@@ -47,6 +47,12 @@ void   or_rhs(const geon_metric_t* m, const double y[8], double out[8]);
 void   or_rk4_step(const geon_metric_t* m, const double y[8], double h, double out[8]);
 void   or_trace(const geon_metric_t* m, const double y0[8], double step,
                 uint64_t max_steps, or_trace_result_t* res);
+/* Adaptive Cash–Karp RK4(5) trace of n rays (no reference counterpart) */
+void   or_ck_step(const geon_metric_t* m, const double y[8], double h, double yn[8],
+                  double err[8]);
+void   or_trace_adaptive_rays(const geon_metric_t* m, const double* y0, size_t n, double h0,
+                              double rtol, double atol, uint64_t max_steps, int32_t* cls,
+                              uint64_t* steps, uint64_t* rejected, double* yf, double* dir);
 /* Batch over n rays (AoS n×8 in, any output may be NULL), nthreads workers
  * with dynamic chunking. Returns elapsed wall seconds. */
 double or_trace_rays(const geon_metric_t* m, const double* y0_aos, size_t n,

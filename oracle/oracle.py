@@ -211,6 +211,25 @@ def trace_rays(m, y0_aos, step, max_steps, nthreads=THREADS):
     return dict(cls=cls, steps=steps, y=yf, resid=resid, dir=d, seconds=el)
 
 
+def trace_rays_adaptive(m, y0_aos, h0, rtol, atol, max_steps):
+    """Adaptive Cash–Karp RK4(5) trace (serial; no reference counterpart)."""
+    y0 = np.ascontiguousarray(y0_aos, dtype=np.float64).reshape(-1, 8)
+    n = y0.shape[0]
+    cls = np.zeros(n, np.int32)
+    steps = np.zeros(n, np.uint64)
+    rej = np.zeros(n, np.uint64)
+    yf = np.zeros((n, 8))
+    d = np.zeros((n, 3))
+    L = lib()
+    L.or_trace_adaptive_rays.argtypes = [C.POINTER(Metric), C.c_void_p, C.c_size_t, C.c_double,
+                                         C.c_double, C.c_double, C.c_uint64, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_trace_adaptive_rays(C.byref(m), y0.ctypes.data, n, h0, rtol, atol, max_steps,
+                             cls.ctypes.data, steps.ctypes.data, rej.ctypes.data,
+                             yf.ctypes.data, d.ctypes.data)
+    return dict(cls=cls, steps=steps, rejected=rej, y=yf, dir=d)
+
+
 def tetrad(m, observer):
     out = np.zeros(16)
     st = lib().or_tetrad(C.byref(m), _dp(_x4(observer)), _dp(out))

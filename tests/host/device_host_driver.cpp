@@ -23,6 +23,23 @@ extern "C" int hd_trace(const geon_metric_t* m, double* y, double h, uint32_t ma
     }
 }
 
+extern "C" int hd_trace_adaptive(const geon_metric_t* m, double* y, double h0, double rtol,
+                                 double atol, uint32_t max_steps, uint32_t* steps,
+                                 uint32_t* rejected) {
+    geon_dmetric P;
+    if (make_dmetric(m, &P) != GEON_OK) return -100;
+    switch (m->kind) {
+        case GEON_METRIC_MINKOWSKI:
+            return trace_adaptive<Minkowski>(P, y, h0, rtol, atol, max_steps, steps, rejected);
+        case GEON_METRIC_SCHWARZSCHILD:
+            return trace_adaptive<Schwarzschild>(P, y, h0, rtol, atol, max_steps, steps, rejected);
+        case GEON_METRIC_ALCUBIERRE:
+            return trace_adaptive<Alcubierre>(P, y, h0, rtol, atol, max_steps, steps, rejected);
+        default:
+            return trace_adaptive<MorrisThorne>(P, y, h0, rtol, atol, max_steps, steps, rejected);
+    }
+}
+
 extern "C" int hd_christoffel(const geon_metric_t* m, const double* x, double* c) {
     geon_dmetric P;
     if (make_dmetric(m, &P) != GEON_OK) return -100;

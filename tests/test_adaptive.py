@@ -1,0 +1,93 @@
+"""Adaptive stepping (Cash–Karp RK4(5)): the step the reference names next
+(integrator.rs:5-7), not a reference method, so its oracle restatement
+(oracle/geon_oracle.c, or_trace_adaptive_rays) is pinned by physics instead:
+straight lines in flat space, conserved energy and angular momentum in
+Schwarzschild, agreement with fine fixed-step traces; and the kernels' code,
+built for the host, equals that restatement bit for bit where the host build
+of the fixed-step path equals the oracle (Minkowski, Schwarzschild,
+Alcubierre)."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from tests.test_device_host import hd, _p  # noqa: F401  (fixture and helper)
+
+
+def _adaptive_host(hd, m, y0, h0, rtol, atol, max_steps):
+    hd.hd_trace_adaptive.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                     C.c_uint32, C.c_void_p, C.c_void_p]
+    Y = np.array(y0, dtype=np.float64)
+    cls, steps, rej = [], [], []
+    for i in range(len(Y)):
+        y = np.ascontiguousarray(Y[i])
+        s, r = C.c_uint32(), C.c_uint32()
+        cls.append(hd.hd_trace_adaptive(C.addressof(m), y.ctypes.data, h0, rtol, atol, max_steps,
+                                        C.addressof(s), C.addressof(r)))
+        steps.append(s.value)
+        rej.append(r.value)
+        Y[i] = y
+    return np.array(cls), np.array(steps), np.array(rej), Y
+
+
+def test_minkowski_lines_are_exact_in_direction(O):
+    m = O.metric(O.MINKOWSKI, escape_radius=100.0)
+    y0 = np.array(O.camera_rays(m, O.camera([0, 20, 0, 0], 60.0, 8, 8)))
+    r = O.trace_rays_adaptive(m, y0, 0.05, 1e-10, 1e-12, 100000)
+    assert (r["cls"] == O.ESCAPED).all() and (r["rejected"] == 0).all()
+    # Γ = 0: every step is the straight line, the momentum never changes
+    assert np.array_equal(r["y"][:, 4:], y0[:, 4:])
+    lam = (r["y"][:, 0] - y0[:, 0]) / y0[:, 4]
+    assert np.abs(r["y"][:, 1:4] - (y0[:, 1:4] + lam[:, None] * y0[:, 5:8])).max() < 1e-10
+
+
+def test_schwarzschild_conserves_energy_and_angular_momentum(O):
+    m = O.metric(O.SCHWARZSCHILD, horizon_pad=1.005, escape_radius=100.0)
+    # a horizontal fan: the rays stay in the equatorial plane with L = r² p^φ ≠ 0
+    y0 = np.array(O.camera_rays(m, O.camera([0, 30, math.pi / 2, 0], 60.0, 24, 1)))
+    assert np.abs(y0[:, 7]).max() > 0 and np.abs(y0[:, 6]).max() == 0
+    r = O.trace_rays_adaptive(m, y0, 0.05, 1e-11, 1e-13, 200000)
+    esc = r["cls"] == O.ESCAPED
+    assert esc.sum() > 10
+    def E(y):
+        return (1 - 2 / y[:, 1]) * y[:, 4]
+    def L(y):
+        return y[:, 1] ** 2 * np.sin(y[:, 2]) ** 2 * y[:, 7]
+    for q in (E, L):
+        a, b = q(y0)[esc], q(r["y"])[esc]
+        assert (np.abs(a - b) <= 1e-8 * np.maximum(1.0, np.abs(a))).all(), q.__name__
+
+
+@pytest.mark.parametrize("kind,kw,obs", [
+    ("schwarzschild", dict(horizon_pad=1.005, escape_radius=100.0), [0, 30, math.pi / 2, 0]),
+    ("alcubierre", dict(bubble_thickness=0.125, ship_velocity=0.5, escape_radius=60.0), [0, 10, 0, 0]),
+    ("morris_thorne", dict(escape_radius=60.0), [0, 15, math.pi / 2, 0]),
+])
+def test_agrees_with_fine_fixed_steps(O, kind, kw, obs):
+    k = {"schwarzschild": O.SCHWARZSCHILD, "alcubierre": O.ALCUBIERRE,
+         "morris_thorne": O.MORRIS_THORNE}[kind]
+    m = O.metric(k, **kw)
+    y0 = np.array(O.camera_rays(m, O.camera(obs, 60.0, 12, 12)))
+    ad = O.trace_rays_adaptive(m, y0, 0.05, 1e-10, 1e-12, 200000)
+    fx = O.trace_rays(m, y0, 0.005, 400000)
+    assert np.array_equal(ad["cls"], fx["cls"])
+    esc = ad["cls"] == O.ESCAPED
+    d = np.clip((ad["dir"][esc] * fx["dir"][esc]).sum(1), -1, 1)
+    assert np.arccos(d).max() < 1e-5
+
+
+@pytest.mark.parametrize("kind,kw,obs", [
+    ("minkowski", dict(escape_radius=100.0), [0, 20, 0, 0]),
+    ("schwarzschild", dict(horizon_pad=1.005, escape_radius=100.0), [0, 30, math.pi / 2, 0]),
+    ("alcubierre", dict(bubble_thickness=0.125, ship_velocity=0.5, escape_radius=60.0), [0, 10, 0, 0]),
+])
+def test_kernel_code_equals_oracle_bit_for_bit(hd, O, kind, kw, obs):
+    k = {"minkowski": O.MINKOWSKI, "schwarzschild": O.SCHWARZSCHILD, "alcubierre": O.ALCUBIERRE}[kind]
+    m = O.metric(k, **kw)
+    y0 = np.array(O.camera_rays(m, O.camera(obs, 60.0, 10, 10)))
+    ref = O.trace_rays_adaptive(m, y0, 0.05, 1e-9, 1e-12, 100000)
+    cls, steps, rej, Y = _adaptive_host(hd, m, y0, 0.05, 1e-9, 1e-12, 100000)
+    assert np.array_equal(cls, ref["cls"]) and np.array_equal(steps, ref["steps"])
+    assert np.array_equal(rej, ref["rejected"])
+    assert np.array_equal(Y.view(np.uint64), ref["y"].view(np.uint64))

@@ -246,6 +246,38 @@ def test_energy_condition_scans_on_gpu(ctx, O, anchors):
     assert abs(mv - want["value"]) <= 1e-4 * abs(want["value"]) and mi == 20
 
 
+@pytest.mark.parametrize("kind,kw,obs", [
+    (0, dict(escape_radius=100.0), (0.0, 20.0, 0.0, 0.0)),
+    (1, dict(horizon_pad=1.005, escape_radius=100.0), (0.0, 30.0, math.pi / 2, 0.0)),
+    (2, dict(bubble_thickness=0.125, ship_velocity=0.5, escape_radius=60.0), (0.0, 10.0, 0.0, 0.0)),
+    (3, dict(escape_radius=60.0), (0.0, 15.0, math.pi / 2, 0.0)),
+])
+def test_adaptive_trace_on_gpu(ctx, O, kind, kw, obs):
+    """geon_trace_rays_adaptive against the oracle's restatement of the same
+    Cash–Karp controller: Minkowski (no transcendental anywhere) bit for bit;
+    elsewhere the library's sin/cos and tanh differ by ulps, so classes must
+    agree and escape directions lie within 1e-8 rad. Morris–Thorne's kernel Γ
+    differs from the oracle's by more ulps (stage sines by rotation); where an
+    error estimate sits at the acceptance threshold the two take different
+    step sequences, each accurate to the method's own error (a few 1e-7 rad
+    against a 10× finer fixed-step trace), so there the bound is 1e-6 rad."""
+    import geon_amd as geon
+    om = O.metric(kind, **kw)
+    y0 = np.array(O.camera_rays(om, O.camera(obs, 60.0, 24, 16)))
+    ref = O.trace_rays_adaptive(om, y0, 0.05, 1e-9, 1e-12, 100000)
+    got = geon.trace_rays_adaptive(om, y0, 0.05, 1e-9, 1e-12, 100000, ctx=ctx)
+    assert np.array_equal(got["cls"], ref["cls"])
+    esc = ref["cls"] == O.ESCAPED
+    ang = _angle(got["dir"][esc], ref["dir"][esc]) if esc.any() else np.zeros(1)
+    print(f"adaptive kind={kind}: rays {len(y0)}, steps {int(got['steps'].sum())} "
+          f"(oracle {int(ref['steps'].sum())}), rejected {int(got['rejected'].sum())}, "
+          f"max angle {ang.max():.3e}", flush=True)
+    if kind == 0:
+        assert np.array_equal(got["y"].view(np.uint64), ref["y"].view(np.uint64))
+        assert np.array_equal(got["steps"], ref["steps"].astype(np.uint32))
+    assert ang.max() < (1e-6 if kind == 3 else 1e-8)
+
+
 def test_shadow_bisection_readme(ctx, O, anchors):
     """validate_shadow: 60 halvings, step 0.01, max 200,000 (README.md:43-45)."""
     import geon_amd as geon
