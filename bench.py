@@ -22,6 +22,9 @@ Also reported on the same run:
                    FP32-mixed server, next to the plain FP64 kernel on the same
                    renders
   curvature        Alcubierre T_μν on 128³ over [-4,4]³, Morris–Thorne on 4096 points
+  adaptive         Cash–Karp RK4(5) traces of the Morris–Thorne camera rays at
+                   512² (host buffers in and out), next to the fixed-step trace
+                   of the same rays through the same kind of entry point
   cpu_baseline     the oracle (reference algorithm in C, -O3, no FMA) on a sample
 """
 from __future__ import annotations
@@ -70,7 +73,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only")
-    ap.add_argument("--sections", default="split,ragged,serving,curvature",
+    ap.add_argument("--sections", default="split,ragged,serving,curvature,adaptive",
                     help="comma-separated extra sections to run after the headline")
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 1 GiB device-to-device copy (HBM counter calibration) and exit")
@@ -192,6 +195,9 @@ def main():
     if "curvature" in sections:
         result["curvature"] = curvature(geon, ctx, torch, stream, ev, alc, mt)
         log(f"curvature: {json.dumps(result['curvature'])}")
+    if "adaptive" in sections:
+        result["adaptive"] = adaptive(geon, jobs)
+        log(f"adaptive: {json.dumps(result['adaptive'])}")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(jobs)
         log(f"cpu baseline: {json.dumps(result['cpu_baseline'])}")
@@ -418,6 +424,44 @@ def curvature(geon, ctx, torch, stream, ev, alc, mt):
         res[label] = {"ms": round(ms, 4), "points_per_s": n / (ms * 1e-3),
                       "ops_per_point": ops[key]["ops"], "achieved_tflops": round(tf, 3),
                       "frac_fp64_peak": round(tf / FP64_PEAK_TFLOPS, 4)}
+    return res
+
+
+def adaptive(geon, jobs):
+    """geon_trace_rays_adaptive on the Morris–Thorne frame's camera rays at
+    512² (h0 = the render's step, at most its max_steps accepted steps, two
+    tolerances), and geon_trace_rays (fixed RK4 steps) on the same rays. Both
+    entry points take host buffers, so both times include the copies
+    (~17 MB in, ~27 MB out). A Cash–Karp step evaluates the right-hand side
+    6 times and an RK4 step 4 times: rhs_per_s compares the kernels."""
+    import numpy as np
+    _, m, cam, cfg = jobs[2]
+    y0 = geon.ray_states(m, geon.Camera(cam.observer, cam.fov_y_deg, 512, 512))
+    n = len(y0)
+    res = {"rays": n, "h0": cfg.step, "max_steps": cfg.max_steps}
+
+    def timed(fn):
+        best, out = None, None
+        for _ in range(3):
+            t = time.perf_counter()
+            out = fn()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        return best, out
+
+    dt, r = timed(lambda: geon.trace_rays(m, y0, cfg.step, cfg.max_steps))
+    st = int(np.sum(r["steps"]))
+    res["fixed_rk4"] = {"ms": round(dt * 1e3, 2), "steps": st, "steps_per_s": st / dt,
+                        "rhs_per_s": 4 * st / dt,
+                        "classes": np.bincount(r["cls"], minlength=4).tolist()}
+    for rtol, atol in ((1e-6, 1e-9), (1e-9, 1e-12)):
+        dt, r = timed(lambda: geon.trace_rays_adaptive(m, y0, cfg.step, rtol, atol,
+                                                       cfg.max_steps))
+        acc, rej = int(np.sum(r["steps"])), int(np.sum(r["rejected"]))
+        res[f"cash_karp_rtol{rtol:g}"] = {
+            "ms": round(dt * 1e3, 2), "accepted": acc, "rejected": rej,
+            "steps_per_s": (acc + rej) / dt, "rhs_per_s": 6 * (acc + rej) / dt,
+            "classes": np.bincount(r["cls"], minlength=4).tolist()}
     return res
 
 

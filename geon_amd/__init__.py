@@ -20,7 +20,7 @@ from ._abi import (ALCUBIERRE, CAPTURED, CURV_EINSTEIN, CURV_RICCI, CURV_RIEMANN
                    PREC_FP64, PREC_MIXED, SCHWARZSCHILD, GeonError, check, lib)
 
 __all__ = ["Minkowski", "Schwarzschild", "Alcubierre", "MorrisThorne", "Camera", "RenderConfig",
-           "Context", "render", "trace_rays", "trace_rays_adaptive", "riemann", "ricci", "ricci_scalar", "einstein",
+           "Context", "render", "ray_states", "trace_rays", "trace_rays_adaptive", "riemann", "ricci", "ricci_scalar", "einstein",
            "stress_energy", "nec_contraction", "nec_scan", "shadow_bisect", "tetrad", "GeonError", "Batch",
            "autotune", "Server", "tonemap_u8", "write_png"]
 
@@ -187,6 +187,25 @@ def tetrad(metric, observer):
     obs = np.ascontiguousarray(observer, dtype=np.float64)
     check(lib().geon_tetrad(C.byref(_cmetric(metric)), _ptr(obs), _ptr(out)), "geon_tetrad")
     return out.reshape(4, 4)
+
+
+def ray_states(metric, camera: Camera, du=0.5, dv=0.5):
+    """Initial states of every pixel of the camera (Camera::tetrad, then
+    Camera::ray at (i + du, j + dv), camera.rs:47-160), AoS (W·H)×8 in
+    row-major pixel order."""
+    tet = tetrad(metric, camera.observer)
+    cam = camera.to_c()
+    n = camera.width * camera.height
+    out = np.zeros((n, 8))
+    f, pc, pt, base = lib().geon_ray_state, C.byref(cam), _ptr(tet), out.ctypes.data
+    du, dv, k = float(du), float(dv), 0
+    for j in range(camera.height):
+        for i in range(camera.width):
+            rc = f(pc, pt, i, j, du, dv, base + 64 * k)
+            if rc:
+                check(rc, "geon_ray_state")
+            k += 1
+    return out
 
 
 def trace_rays(metric, y0, step, max_steps, ctx: Context | None = None):

@@ -388,9 +388,10 @@ void or_rk4_step(const geon_metric_t* m, const double y[8], double h, double out
  * 201-222) on or_rhs, the error test max_i |err_i|/(atol + rtol·max(|y_i|,
  * |y_new_i|)) ≤ 1, next step h·0.9·e^(-1/4) in [h/10, 5h] (5h below
  * e = 1.0498e-3), |h| in [1e-8, 8]·|h0|; every sum left to right in the
- * tableau's order. Parity unpinned against any reference: the tests pin it
- * by closed forms (straight lines, conserved energy and angular momentum)
- * and against fine fixed-step traces of or_trace. */
+ * tableau's order. No reference result exists for it; the tests pin it by
+ * the tableau's orders (local errors O(h^6) and O(h^5) of the fifth- and
+ * embedded fourth-order solutions), closed forms (straight lines, conserved
+ * energy and angular momentum) and fine fixed-step traces of or_trace. */
 void or_ck_step(const geon_metric_t* m, const double y[8], double h, double yn[8], double err[8]) {
     const double a21 = 1.0 / 5.0;
     const double a31 = 3.0 / 40.0, a32 = 9.0 / 40.0;

@@ -211,6 +211,18 @@ def trace_rays(m, y0_aos, step, max_steps, nthreads=THREADS):
     return dict(cls=cls, steps=steps, y=yf, resid=resid, dir=d, seconds=el)
 
 
+def ck_step(m, y, h):
+    """One Cash–Karp step: (fifth-order solution, embedded error estimate)."""
+    L = lib()
+    L.or_ck_step.argtypes = [C.POINTER(Metric), C.c_void_p, C.c_double, C.c_void_p, C.c_void_p]
+    L.or_ck_step.restype = None
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    yn = np.zeros(8)
+    err = np.zeros(8)
+    L.or_ck_step(C.byref(m), y.ctypes.data, h, yn.ctypes.data, err.ctypes.data)
+    return yn, err
+
+
 def trace_rays_adaptive(m, y0_aos, h0, rtol, atol, max_steps):
     """Adaptive Cash–Karp RK4(5) trace (serial; no reference counterpart)."""
     y0 = np.ascontiguousarray(y0_aos, dtype=np.float64).reshape(-1, 8)

@@ -42,6 +42,38 @@ def test_minkowski_lines_are_exact_in_direction(O):
     assert np.abs(r["y"][:, 1:4] - (y0[:, 1:4] + lam[:, None] * y0[:, 5:8])).max() < 1e-10
 
 
+def test_cash_karp_orders(O):
+    """The tableau's orders, which pin its constants: a wrong coefficient
+    drops the local error of the fifth-order solution from O(h⁶) or that of
+    the embedded fourth-order one from O(h⁵). Local errors against the same
+    step taken as 256 substeps (error ~256·(h/256)⁶, negligible), on a bent
+    Schwarzschild orbit near r = 4M."""
+    m = O.metric(O.SCHWARZSCHILD, horizon_pad=1.005, escape_radius=100.0)
+    r0 = 4.0
+    pt = 1.0 / (1 - 2 / r0)
+    pphi = 0.3 / r0 ** 2
+    pr = -math.sqrt((1 - 2 / r0) * ((1 - 2 / r0) * pt ** 2 - r0 ** 2 * pphi ** 2))
+    y0 = np.array([0.0, r0, math.pi / 2, 0.0, pt, pr, 0.0, pphi])
+
+    def fine(h, n=256):
+        y = y0.copy()
+        for _ in range(n):
+            y, _ = O.ck_step(m, y, h / n)
+        return y
+
+    e5, est = [], []
+    for h in (0.4, 0.2, 0.1):
+        yn, err = O.ck_step(m, y0, h)
+        e5.append(np.abs(yn - fine(h)).max())
+        est.append(np.abs(err).max())
+    # halving h divides the local error by 2⁶ and the estimate (the fourth-
+    # order solution's local error) by 2⁵, up to the next terms
+    for a, b in zip(e5, e5[1:]):
+        assert 2 ** 5.3 < a / b < 2 ** 6.7, e5
+    for a, b in zip(est, est[1:]):
+        assert 2 ** 4.5 < a / b < 2 ** 5.5, est
+
+
 def test_schwarzschild_conserves_energy_and_angular_momentum(O):
     m = O.metric(O.SCHWARZSCHILD, horizon_pad=1.005, escape_radius=100.0)
     # a horizontal fan: the rays stay in the equatorial plane with L = r² p^φ ≠ 0
