@@ -23,8 +23,8 @@ Also reported on the same run:
                    renders
   curvature        Alcubierre T_μν on 128³ over [-4,4]³, Morris–Thorne on 4096 points
   adaptive         Cash–Karp RK4(5) traces of the Morris–Thorne camera rays at
-                   512² (host buffers in and out), next to the fixed-step trace
-                   of the same rays through the same kind of entry point
+                   512², next to the fixed-step trace of the same rays (both on
+                   device-resident states)
   cpu_baseline     the oracle (reference algorithm in C, -O3, no FMA) on a sample
 """
 from __future__ import annotations
@@ -196,7 +196,7 @@ def main():
         result["curvature"] = curvature(geon, ctx, torch, stream, ev, alc, mt)
         log(f"curvature: {json.dumps(result['curvature'])}")
     if "adaptive" in sections:
-        result["adaptive"] = adaptive(geon, jobs)
+        result["adaptive"] = adaptive(geon, ctx, torch, stream, ev, jobs)
         log(f"adaptive: {json.dumps(result['adaptive'])}")
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(jobs)
@@ -427,41 +427,63 @@ def curvature(geon, ctx, torch, stream, ev, alc, mt):
     return res
 
 
-def adaptive(geon, jobs):
-    """geon_trace_rays_adaptive on the Morris–Thorne frame's camera rays at
-    512² (h0 = the render's step, at most its max_steps accepted steps, two
-    tolerances), and geon_trace_rays (fixed RK4 steps) on the same rays. Both
-    entry points take host buffers, so both times include the copies
-    (~17 MB in, ~27 MB out). A Cash–Karp step evaluates the right-hand side
-    6 times and an RK4 step 4 times: rhs_per_s compares the kernels."""
+def adaptive(geon, ctx, torch, stream, ev, jobs):
+    """geon_trace_rays_adaptive_dev on the Morris–Thorne frame's camera rays
+    at 512² (h0 = the render's step, at most its max_steps accepted steps,
+    two tolerances), and geon_trace_rays_dev (fixed RK4 steps) on the same
+    rays; states resident in HBM, HIP events on the launch stream. A
+    Cash–Karp step evaluates the right-hand side 6 times and an RK4 step 4
+    times: rhs_per_s compares the kernels."""
     import numpy as np
     _, m, cam, cfg = jobs[2]
     y0 = geon.ray_states(m, geon.Camera(cam.observer, cam.fov_y_deg, 512, 512))
     n = len(y0)
-    res = {"rays": n, "h0": cfg.step, "max_steps": cfg.max_steps}
+    dy0 = torch.from_numpy(np.ascontiguousarray(y0.T)).to("cuda")
+    cls = torch.empty(n, dtype=torch.int32, device="cuda")
+    steps = torch.empty(n, dtype=torch.int32, device="cuda")
+    rej = torch.empty(n, dtype=torch.int32, device="cuda")
+    yf = torch.empty(8 * n, dtype=torch.float64, device="cuda")
+    dirs = torch.empty(3 * n, dtype=torch.float64, device="cuda")
+    cm = geon.C.byref(geon._cmetric(m))
+    vp = geon.C.c_void_p
+    torch.cuda.synchronize()   # the states' copy ran on torch's stream
+    res = {"workload": "Morris-Thorne b0=1 camera rays, l=15, 60°, 512²", "rays": n,
+           "h0": cfg.step, "max_steps": cfg.max_steps}
 
-    def timed(fn):
-        best, out = None, None
+    def timed(launch):
+        times = []
         for _ in range(3):
-            t = time.perf_counter()
-            out = fn()
-            dt = time.perf_counter() - t
-            best = dt if best is None else min(best, dt)
-        return best, out
+            a, b = ev(), ev()
+            a.record(stream)
+            launch()
+            b.record(stream)
+            b.synchronize()
+            times.append(a.elapsed_time(b))
+        return min(times)
 
-    dt, r = timed(lambda: geon.trace_rays(m, y0, cfg.step, cfg.max_steps))
-    st = int(np.sum(r["steps"]))
-    res["fixed_rk4"] = {"ms": round(dt * 1e3, 2), "steps": st, "steps_per_s": st / dt,
-                        "rhs_per_s": 4 * st / dt,
-                        "classes": np.bincount(r["cls"], minlength=4).tolist()}
+    def fixed():
+        geon._abi.check(geon.lib().geon_trace_rays_dev(
+            ctx.handle, cm, vp(dy0.data_ptr()), n, cfg.step, cfg.max_steps, vp(cls.data_ptr()),
+            vp(steps.data_ptr()), vp(yf.data_ptr()), None, vp(dirs.data_ptr()),
+            vp(stream.cuda_stream)), "geon_trace_rays_dev")
+
+    ms = timed(fixed)
+    st = int(steps.to(torch.int64).sum())
+    res["fixed_rk4"] = {"ms": round(ms, 3), "steps": st, "steps_per_s": st / (ms * 1e-3),
+                        "rhs_per_s": 4 * st / (ms * 1e-3),
+                        "classes": torch.bincount(cls, minlength=4).tolist()}
     for rtol, atol in ((1e-6, 1e-9), (1e-9, 1e-12)):
-        dt, r = timed(lambda: geon.trace_rays_adaptive(m, y0, cfg.step, rtol, atol,
-                                                       cfg.max_steps))
-        acc, rej = int(np.sum(r["steps"])), int(np.sum(r["rejected"]))
+        def run():
+            geon._abi.check(geon.lib().geon_trace_rays_adaptive_dev(
+                ctx.handle, cm, vp(dy0.data_ptr()), n, cfg.step, rtol, atol, cfg.max_steps,
+                vp(cls.data_ptr()), vp(steps.data_ptr()), vp(rej.data_ptr()), vp(yf.data_ptr()),
+                vp(dirs.data_ptr()), vp(stream.cuda_stream)), "geon_trace_rays_adaptive_dev")
+        ms = timed(run)
+        acc, rj = int(steps.to(torch.int64).sum()), int(rej.to(torch.int64).sum())
         res[f"cash_karp_rtol{rtol:g}"] = {
-            "ms": round(dt * 1e3, 2), "accepted": acc, "rejected": rej,
-            "steps_per_s": (acc + rej) / dt, "rhs_per_s": 6 * (acc + rej) / dt,
-            "classes": np.bincount(r["cls"], minlength=4).tolist()}
+            "ms": round(ms, 3), "accepted": acc, "rejected": rj,
+            "steps_per_s": (acc + rj) / (ms * 1e-3), "rhs_per_s": 6 * (acc + rj) / (ms * 1e-3),
+            "classes": torch.bincount(cls, minlength=4).tolist()}
     return res
 
 

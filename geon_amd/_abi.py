@@ -97,6 +97,8 @@ SIGNATURES = [
     ("geon_nec_contraction", _i, [_vp, C.POINTER(Metric), _vp, _sz, _vp, _vp]),
     ("geon_trace_rays_adaptive", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_double, C.c_double,
                                       C.c_double, C.c_uint32, _vp, _vp, _vp, _vp, _vp]),
+    ("geon_trace_rays_adaptive_dev", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_double, C.c_double,
+                                          C.c_double, C.c_uint32, _vp, _vp, _vp, _vp, _vp, _vp]),
     ("geon_nec_contraction_dev", _i, [_vp, C.POINTER(Metric), _vp, _sz, _vp, _vp, _vp]),
     ("geon_nec_scan", _i, [_vp, C.POINTER(Metric), _vp, _sz, _vp, C.c_double, _vp, _vp, _vp, _vp]),
     ("geon_shadow_bisect", _i, [_vp, _vp, _vp, _sz, C.c_double, _u32, _u32, _vp, _vp]),

@@ -159,12 +159,15 @@ int geon_trace_rays(geon_ctx_t* ctx, const geon_metric_t* metric, const double* 
     return GEON_OK;
 }
 
-int geon_trace_rays_adaptive(geon_ctx_t* ctx, const geon_metric_t* metric, const double* y0_soa,
-                             size_t n, double h0, double rtol, double atol, uint32_t max_steps,
-                             int32_t* cls, uint32_t* steps, uint32_t* rejected, double* y_final_soa,
-                             double* dir_soa) {
-    if (!ctx || !metric || !y0_soa || !(h0 != 0.0 && isfinite(h0)) || !(rtol >= 0.0 && isfinite(rtol)) ||
-        !(atol > 0.0 && isfinite(atol)))
+static bool adaptive_args_ok(double h0, double rtol, double atol) {
+    return h0 != 0.0 && isfinite(h0) && rtol >= 0.0 && isfinite(rtol) && atol > 0.0 && isfinite(atol);
+}
+
+int geon_trace_rays_adaptive_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const double* y0_soa,
+                                 size_t n, double h0, double rtol, double atol, uint32_t max_steps,
+                                 int32_t* cls, uint32_t* steps, uint32_t* rejected,
+                                 double* y_final_soa, double* dir_soa, void* stream) {
+    if (!ctx || !metric || !y0_soa || !adaptive_args_ok(h0, rtol, atol))
         return fail(GEON_E_INVALID, "bad argument");
     if (n == 0) return GEON_OK;
     hipSetDevice(ctx->device);
@@ -172,6 +175,24 @@ int geon_trace_rays_adaptive(geon_ctx_t* ctx, const geon_metric_t* metric, const
     GEON_TRY(fill_frame(metric, &F));
     F.step = h0;
     F.max_steps = max_steps;
+    hipStream_t s = pick_stream(ctx, stream);
+    uint32_t slot;
+    GEON_TRY(ctx_slot(ctx, &F, s, &slot));
+    uint64_t nn = n;
+    void* args[] = {&slot, &nn, (void*)&y0_soa, &rtol, &atol, &cls, &steps, &rejected, &y_final_soa,
+                    &dir_soa};
+    GEON_TRY(launch(ctx->fn_trace_adaptive[metric->kind], nn, 256, s, args));
+    return ctx_slot_done(ctx, slot, s);
+}
+
+int geon_trace_rays_adaptive(geon_ctx_t* ctx, const geon_metric_t* metric, const double* y0_soa,
+                             size_t n, double h0, double rtol, double atol, uint32_t max_steps,
+                             int32_t* cls, uint32_t* steps, uint32_t* rejected, double* y_final_soa,
+                             double* dir_soa) {
+    if (!ctx || !metric || !y0_soa || !adaptive_args_ok(h0, rtol, atol))
+        return fail(GEON_E_INVALID, "bad argument");
+    if (n == 0) return GEON_OK;
+    hipSetDevice(ctx->device);
     void *d_y0, *d_cls = nullptr, *d_steps = nullptr, *d_rej = nullptr, *d_yf = nullptr, *d_dir = nullptr;
     GEON_TRY(ctx_buffer(ctx, "ad_y0", n * 64, &d_y0));
     if (cls) GEON_TRY(ctx_buffer(ctx, "ad_cls", n * 4, &d_cls));
@@ -181,12 +202,9 @@ int geon_trace_rays_adaptive(geon_ctx_t* ctx, const geon_metric_t* metric, const
     if (dir_soa) GEON_TRY(ctx_buffer(ctx, "ad_dir", n * 24, &d_dir));
     hipStream_t s = ctx->stream;
     GEON_HIP(hipMemcpyHtoDAsync((hipDeviceptr_t)d_y0, (void*)y0_soa, n * 64, s), "H2D");
-    uint32_t slot;
-    GEON_TRY(ctx_slot(ctx, &F, s, &slot));
-    uint64_t nn = n;
-    void* args[] = {&slot, &nn, &d_y0, &rtol, &atol, &d_cls, &d_steps, &d_rej, &d_yf, &d_dir};
-    GEON_TRY(launch(ctx->fn_trace_adaptive[metric->kind], nn, 256, s, args));
-    GEON_TRY(ctx_slot_done(ctx, slot, s));
+    GEON_TRY(geon_trace_rays_adaptive_dev(ctx, metric, (const double*)d_y0, n, h0, rtol, atol,
+                                          max_steps, (int32_t*)d_cls, (uint32_t*)d_steps,
+                                          (uint32_t*)d_rej, (double*)d_yf, (double*)d_dir, s));
     if (cls) GEON_HIP(hipMemcpyDtoHAsync(cls, (hipDeviceptr_t)d_cls, n * 4, s), "D2H");
     if (steps) GEON_HIP(hipMemcpyDtoHAsync(steps, (hipDeviceptr_t)d_steps, n * 4, s), "D2H");
     if (rejected) GEON_HIP(hipMemcpyDtoHAsync(rejected, (hipDeviceptr_t)d_rej, n * 4, s), "D2H");

@@ -14,7 +14,7 @@
  *                             src/metric/minkowski.rs:8)
  *   geon_tetrad              Camera::tetrad            (src/camera.rs:47-97)
  *   geon_trace_rays[_dev]    geodesic::trace, batched  (src/geodesic.rs:62-117)
- *   geon_trace_rays_adaptive no counterpart: the adaptive stepping the
+ *   geon_trace_rays_adaptive[_dev] no counterpart: the adaptive stepping the
  *                            reference names as its next step
  *                            (src/integrator.rs:5-7), same trace semantics
  *   geon_render[_dev]        render::render            (src/render.rs:55-165)
@@ -203,6 +203,11 @@ int  geon_trace_rays_adaptive(geon_ctx_t* ctx, const geon_metric_t* metric, cons
                               size_t n, double h0, double rtol, double atol, uint32_t max_steps,
                               int32_t* cls, uint32_t* steps, uint32_t* rejected,
                               double* y_final_soa, double* dir_soa);
+int  geon_trace_rays_adaptive_dev(geon_ctx_t* ctx, const geon_metric_t* metric,
+                                  const double* y0_soa, size_t n, double h0, double rtol,
+                                  double atol, uint32_t max_steps, int32_t* cls, uint32_t* steps,
+                                  uint32_t* rejected, double* y_final_soa, double* dir_soa,
+                                  void* stream);
 
 /* Curvature at n points x (SoA 4×n). out: n×{256|16|1} doubles. */
 int  geon_curvature(geon_ctx_t* ctx, const geon_metric_t* metric, const double* x_soa,

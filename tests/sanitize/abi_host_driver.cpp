@@ -60,6 +60,13 @@ int main() {
                   "nec contraction without context");
     bad += expect(geon_nec_scan(nullptr, &mt, xs, 1, kv, 0.05, nullptr, &vmin, &imin, &integral),
                   GEON_E_INVALID, "nec scan without context");
+    const double y8[8] = {0.0, 15.0, 1.5707963267948966, 0.0, 1.0, -1.0, 0.0, 0.0};
+    bad += expect(geon_trace_rays_adaptive(nullptr, &mt, y8, 1, 0.05, 1e-9, 1e-12, 10, nullptr,
+                                           nullptr, nullptr, nullptr, nullptr),
+                  GEON_E_INVALID, "adaptive trace without context");
+    bad += expect(geon_trace_rays_adaptive_dev(nullptr, &mt, y8, 1, 0.0, 1e-9, 1e-12, 10, nullptr,
+                                               nullptr, nullptr, nullptr, nullptr, nullptr),
+                  GEON_E_INVALID, "adaptive trace with h0 = 0");
     printf("driver done, failures %d\n", bad);
     return bad ? 1 : 0;
 }

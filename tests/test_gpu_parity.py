@@ -277,6 +277,31 @@ def test_adaptive_trace_on_gpu(ctx, O, kind, kw, obs):
         assert np.array_equal(got["steps"], ref["steps"].astype(np.uint32))
     assert ang.max() < (1e-6 if kind == 3 else 1e-8)
 
+    # the device-pointer entry point on torch's stream: the same kernel, the same bits
+    import torch
+    n = len(y0)
+    dy0 = torch.from_numpy(np.ascontiguousarray(y0.T)).cuda()
+    dcls = torch.full((n,), -7, dtype=torch.int32, device="cuda")
+    dsteps = torch.zeros(n, dtype=torch.int32, device="cuda")
+    drej = torch.zeros(n, dtype=torch.int32, device="cuda")
+    dyf = torch.zeros(8 * n, dtype=torch.float64, device="cuda")
+    ddir = torch.zeros(3 * n, dtype=torch.float64, device="cuda")
+    # a stream of the caller's own (torch's default stream is the null handle,
+    # which the library takes as "use the context's stream")
+    st = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    geon._abi.check(geon.lib().geon_trace_rays_adaptive_dev(
+        ctx.handle, geon.C.byref(geon._cmetric(om)), dy0.data_ptr(), n, 0.05, 1e-9, 1e-12, 100000,
+        dcls.data_ptr(), dsteps.data_ptr(), drej.data_ptr(), dyf.data_ptr(), ddir.data_ptr(),
+        st.cuda_stream), "geon_trace_rays_adaptive_dev")
+    st.synchronize()
+    assert np.array_equal(dcls.cpu().numpy(), got["cls"])
+    assert np.array_equal(dsteps.cpu().numpy().view(np.uint32), got["steps"])
+    assert np.array_equal(drej.cpu().numpy().view(np.uint32), got["rejected"])
+    assert np.array_equal(dyf.cpu().numpy().reshape(8, n).T.view(np.uint64), got["y"].view(np.uint64))
+    assert np.array_equal(ddir.cpu().numpy().reshape(3, n).T.view(np.uint64),
+                          got["dir"].view(np.uint64))
+
 
 def test_shadow_bisection_readme(ctx, O, anchors):
     """validate_shadow: 60 halvings, step 0.01, max 200,000 (README.md:43-45)."""
