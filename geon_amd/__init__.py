@@ -100,18 +100,23 @@ def _cmetric(m):
 
 @dataclass
 class Camera:
-    """Camera::new(observer, fov_y_deg, width, height) (camera.rs:29-36)."""
+    """Camera::new(observer, fov_y_deg, width, height) (camera.rs:29-36): the
+    field of view is given in degrees and held in radians (``fov_y_rad``), as
+    the reference's Camera holds it."""
     observer: tuple
     fov_y_deg: float
     width: int
     height: int
 
+    @property
+    def fov_y_rad(self) -> float:
+        return self.to_c().fov_y_rad
+
     def to_c(self) -> _abi.Camera:
         c = _abi.Camera()
-        for i in range(4):
-            c.observer[i] = float(self.observer[i])
-        c.fov_y_deg = float(self.fov_y_deg)
-        c.width, c.height = int(self.width), int(self.height)
+        obs = (C.c_double * 4)(*[float(v) for v in self.observer])
+        check(lib().geon_camera_new(obs, float(self.fov_y_deg), int(self.width), int(self.height),
+                                    C.byref(c)), "geon_camera_new")
         return c
 
 

@@ -37,7 +37,7 @@ class Metric(C.Structure):
 
 
 class Camera(C.Structure):
-    _fields_ = [("observer", C.c_double * 4), ("fov_y_deg", C.c_double),
+    _fields_ = [("observer", C.c_double * 4), ("fov_y_rad", C.c_double),
                 ("width", C.c_uint32), ("height", C.c_uint32)]
 
 
@@ -62,7 +62,8 @@ class RenderStats(C.Structure):
     _fields_ = [("captured", C.c_uint64), ("escaped", C.c_uint64), ("diverged", C.c_uint64),
                 ("max_null_residual", C.c_double), ("avg_steps", C.c_double),
                 ("max_steps_hit", C.c_uint64), ("ray_steps", C.c_uint64),
-                ("fp64_rays", C.c_uint64), ("kernel_ms", C.c_double)]
+                ("fp64_rays", C.c_uint64), ("kernel_ms", C.c_double),
+                ("gamma_steps", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -80,6 +81,13 @@ SIGNATURES = [
     ("geon_last_error", C.c_char_p, []),
     ("geon_version", _i, []),
     ("geon_metric_default", _i, [C.c_int32, C.POINTER(Metric)]),
+    ("geon_metric_g", _i, [C.POINTER(Metric), _vp, _vp]),
+    ("geon_metric_dg", _i, [C.POINTER(Metric), _vp, _vp]),
+    ("geon_metric_g_inv", _i, [C.POINTER(Metric), _vp, _vp]),
+    ("geon_metric_christoffel", _i, [C.POINTER(Metric), _vp, _vp]),
+    ("geon_metric_terminate", _i, [C.POINTER(Metric), _vp, C.POINTER(C.c_int32)]),
+    ("geon_metric_asymptotic_direction", _i, [C.POINTER(Metric), _vp, _vp, _vp]),
+    ("geon_camera_new", _i, [_vp, C.c_double, _u32, _u32, C.POINTER(Camera)]),
     ("geon_ctx_create", _i, [_i, C.c_char_p, C.POINTER(_vp)]),
     ("geon_ctx_destroy", None, [_vp]),
     ("geon_tetrad", _i, [C.POINTER(Metric), _vp, _vp]),

@@ -28,8 +28,9 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 # Packed FP32 instructions are disabled for every HIP translation unit.
 HIP_FLAGS = ["--genco", "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
              "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden",
-              "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-result"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-shared", "-fvisibility=hidden", "-ffp-contract=off",
+              "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include", "-Wall", "-Wno-unused-result",
+              "-Wno-unknown-pragmas"]
 
 KERNEL_SRC = [os.path.join(CSRC, "kernels", "geon_kernels.hip")]
 KERNEL_DEPS = KERNEL_SRC + [os.path.join(CSRC, "kernels", f)
@@ -37,7 +38,8 @@ KERNEL_DEPS = KERNEL_SRC + [os.path.join(CSRC, "kernels", f)
                                       "geon_curv.hpp")] + \
     [os.path.join(CSRC, "geon_dev_types.h")]
 HOST_SRC = [os.path.join(CSRC, "host", f)
-            for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp", "geon_serve.cpp")]
+            for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp", "geon_serve.cpp",
+                      "geon_metric_host.cpp")]
 
 
 def _run(cmd):
@@ -59,7 +61,7 @@ def build_host(out=None):
     os.makedirs(os.path.dirname(out), exist_ok=True)
     tmp = out + ".tmp"
     cxx = shutil.which("g++") or "g++"
-    _run([cxx, *HOST_FLAGS, *HOST_SRC, f"-L{ROCM}/lib", "-lamdhip64", "-lpthread",
+    _run([cxx, *HOST_FLAGS, *HOST_SRC, f"-L{ROCM}/lib", "-lamdhip64", "-lpthread", "-lm",
           f"-Wl,-rpath,{ROCM}/lib",
           "-o", tmp])
     os.replace(tmp, out)

@@ -56,7 +56,9 @@ struct geon_dstats {
     unsigned long long max_steps_hit;
     unsigned long long ray_steps;
     unsigned long long fp64_rays;
+    unsigned long long gamma_steps;      // steps that evaluated Γ (plain renders)
 };
+#define GEON_DSTATS_WORDS (sizeof(struct geon_dstats) / sizeof(unsigned long long))
 
 // One unit of scheduled work: a camera frame (gen 0) or a set of explicit
 // initial states (gen 1). Rays of a queue are numbered globally; this job

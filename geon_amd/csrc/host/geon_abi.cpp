@@ -29,9 +29,8 @@ int fill_camera(const geon_metric_t* metric, const geon_camera_t* cam, geon_dfra
         return fail(GEON_E_INVALID, "camera has zero size");
     GEON_TRY(tetrad(metric, cam->observer, F->tet));
     memcpy(F->obs, cam->observer, sizeof F->obs);
-    const double fov_rad = cam->fov_y_deg * (kPi / 180.0);
     const double aspect = (double)cam->width / (double)cam->height;
-    F->h_im = tan(fov_rad * 0.5);
+    F->h_im = tan(cam->fov_y_rad * 0.5);
     F->w_im = F->h_im * aspect;
     F->W = cam->width;
     F->H = cam->height;
@@ -82,11 +81,27 @@ int geon_metric_default(int32_t kind, geon_metric_t* out) {
     return GEON_OK;
 }
 
+// Camera::new: f64::to_radians is deg·(π/180) (camera.rs:29-36).
+int geon_camera_new(const double observer[4], double fov_y_deg, uint32_t width, uint32_t height,
+                    geon_camera_t* out) {
+    if (!observer || !out) return fail(GEON_E_INVALID, "bad argument");
+    memcpy(out->observer, observer, sizeof out->observer);
+    out->fov_y_rad = fov_y_deg * (kPi / 180.0);
+    out->width = width;
+    out->height = height;
+    return GEON_OK;
+}
+
 int geon_ctx_create(int device, const char* code_object, geon_ctx_t** out) {
     return ctx_create(device, code_object, out);
 }
 
-void geon_ctx_destroy(geon_ctx_t* ctx) { ctx_destroy(ctx); }
+void geon_ctx_destroy(geon_ctx_t* ctx) {
+    if (!ctx) return;
+    hipSetDevice(ctx->device);
+    for (auto& kv : ctx->lanes) lane_reap(kv.second, true);
+    ctx_destroy(ctx);
+}
 
 int geon_tetrad(const geon_metric_t* metric, const double observer[4], double tetrad_out[16]) {
     if (!metric_valid(metric) || !observer || !tetrad_out) return fail(GEON_E_INVALID, "bad argument");
@@ -97,9 +112,8 @@ int geon_ray_state(const geon_camera_t* cam, const double tet[16], uint32_t i, u
                    double du, double dv, double y[8]) {
     if (!cam || !tet || !y || cam->width == 0 || cam->height == 0)
         return fail(GEON_E_INVALID, "bad argument");
-    const double fov_rad = cam->fov_y_deg * (kPi / 180.0);
     const double aspect = (double)cam->width / (double)cam->height;
-    const double h = tan(fov_rad * 0.5);
+    const double h = tan(cam->fov_y_rad * 0.5);
     const double w = h * aspect;
     const double u = (2.0 * ((double)i + du) / (double)cam->width - 1.0) * w;
     const double v = -(2.0 * ((double)j + dv) / (double)cam->height - 1.0) * h;
@@ -218,20 +232,28 @@ int geon_render_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_cam
                     const geon_render_cfg_t* cfg, float* rgb, int32_t* cls_per_pixel,
                     double* dir_per_pixel, geon_render_stats_t* stats, void* stream) {
     if (!ctx || !cfg) return fail(GEON_E_INVALID, "bad argument");
+    hipSetDevice(ctx->device);
+    hipStream_t s = pick_stream(ctx, stream);
+    geon_lane* lane;
+    GEON_TRY(ctx_lane(ctx, s, &lane));
+    lane_reap(lane, false);
     if (cfg->precision == GEON_PREC_MIXED) {
-        // FP32-mixed renders run on the persistent scheduler
+        // FP32-mixed renders run on the persistent scheduler. The batch's
+        // buffers live until its work on `s` has finished: with statistics
+        // requested the call waits for it anyway; otherwise the lane keeps
+        // the batch and frees it once its end event has passed.
         geon_batch_t* b = nullptr;
         GEON_TRY(geon_batch_create(ctx, &b));
         uint32_t fid = 0;
         int st = geon_batch_add_frame(b, metric, cam, cfg, rgb, cls_per_pixel, dir_per_pixel, &fid);
-        if (st == GEON_OK) st = geon_batch_run(b, nullptr, stream);
+        if (st == GEON_OK) st = geon_batch_run(b, nullptr, s);
         if (st == GEON_OK && stats) st = geon_batch_frame_stats(b, fid, stats);
-        geon_batch_destroy(b);
+        if (st == GEON_OK && !stats) lane->retired.push_back(b);
+        else geon_batch_destroy(b);
         return st;
     }
     if (cfg->precision != GEON_PREC_FP64)
         return fail(GEON_E_INVALID, "unknown precision %d", cfg->precision);
-    hipSetDevice(ctx->device);
     geon_dframe F;
     GEON_TRY(fill_frame(metric, &F));
     GEON_TRY(fill_camera(metric, cam, &F));
@@ -247,30 +269,30 @@ int geon_render_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_cam
     const uint64_t n_pix = (uint64_t)cam->width * cam->height;
     const uint64_t n_rays = n_pix * spa * spa;
     void *d_cls, *d_steps, *d_res, *d_dir;
-    GEON_TRY(ctx_buffer(ctx, "ray_cls", n_rays * 4, &d_cls));
-    GEON_TRY(ctx_buffer(ctx, "ray_steps", n_rays * 4, &d_steps));
-    GEON_TRY(ctx_buffer(ctx, "ray_res", n_rays * 8, &d_res));
-    GEON_TRY(ctx_buffer(ctx, "ray_dir", n_rays * 24, &d_dir));
-    hipStream_t s = pick_stream(ctx, stream);
+    GEON_TRY(lane_buffer(lane, "ray_cls", n_rays * 4, &d_cls));
+    GEON_TRY(lane_buffer(lane, "ray_steps", n_rays * 4, &d_steps));
+    GEON_TRY(lane_buffer(lane, "ray_res", n_rays * 8, &d_res));
+    GEON_TRY(lane_buffer(lane, "ray_dir", n_rays * 24, &d_dir));
     uint32_t slot;
     GEON_TRY(ctx_slot(ctx, &F, s, &slot));
-    void* cargs[] = {&ctx->d_stats};
+    geon_dstats* d_stats = lane->d_stats;
+    void* cargs[] = {&d_stats};
     GEON_HIP(hipModuleLaunchKernel(ctx->fn_stats_clear, 1, 1, 1, 64, 1, 1, 0, s, cargs, nullptr),
              "stats clear");
-    GEON_HIP(hipEventRecord(ctx->t0, s), "event");
-    void* rargs[] = {&slot, (void*)&n_rays, &d_cls, &d_steps, &d_res, &d_dir};
+    GEON_HIP(hipEventRecord(lane->t0, s), "event");
+    void* rargs[] = {&slot, (void*)&n_rays, &d_cls, &d_steps, &d_res, &d_dir, &d_stats};
     GEON_TRY(launch(ctx->fn_render[metric->kind], n_rays, 256, s, rargs));
-    GEON_HIP(hipEventRecord(ctx->t1, s), "event");
+    GEON_HIP(hipEventRecord(lane->t1, s), "event");
     void* fargs[] = {&slot, (void*)&n_pix, &d_cls, &d_steps, &d_res, &d_dir,
-                     &rgb, &cls_per_pixel, &dir_per_pixel, &ctx->d_stats};
+                     &rgb, &cls_per_pixel, &dir_per_pixel, &d_stats};
     GEON_TRY(launch(ctx->fn_finalize, n_pix, 256, s, fargs));
     GEON_TRY(ctx_slot_done(ctx, slot, s));
     if (stats) {
         geon_dstats ds;
-        GEON_HIP(hipMemcpyDtoHAsync(&ds, (hipDeviceptr_t)ctx->d_stats, sizeof ds, s), "D2H stats");
+        GEON_HIP(hipMemcpyDtoHAsync(&ds, (hipDeviceptr_t)d_stats, sizeof ds, s), "D2H stats");
         GEON_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
         float ms = 0.0f;
-        hipEventElapsedTime(&ms, ctx->t0, ctx->t1);
+        hipEventElapsedTime(&ms, lane->t0, lane->t1);
         memset(stats, 0, sizeof *stats);
         stats->captured = ds.captured;
         stats->escaped = ds.escaped;
@@ -283,6 +305,7 @@ int geon_render_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_cam
         stats->ray_steps = ds.ray_steps;
         stats->fp64_rays = n_rays;
         stats->kernel_ms = ms;
+        stats->gamma_steps = ds.gamma_steps;
     }
     return GEON_OK;
 }
@@ -323,8 +346,11 @@ int geon_curvature_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const doubl
     uint64_t nn = n;
     double factor = 1.0 / (8.0 * kPi);  // curvature.rs:118
     void* scratch = nullptr;
-    if (what == GEON_CURV_RIEMANN && metric->kind != GEON_METRIC_ALCUBIERRE)
-        GEON_TRY(ctx_buffer(ctx, "curv_scratch", n * 256 * 8, &scratch));
+    if (what == GEON_CURV_RIEMANN && metric->kind != GEON_METRIC_ALCUBIERRE) {
+        geon_lane* lane;
+        GEON_TRY(ctx_lane(ctx, s, &lane));
+        GEON_TRY(lane_buffer(lane, "curv_scratch", n * 256 * 8, &scratch));
+    }
     void* args[] = {&slot, &nn, (void*)&x_soa, &what, &out, &scratch, &factor};
     if (metric->kind == GEON_METRIC_ALCUBIERRE)   // one 8-lane team per point, 8 points per block
         GEON_TRY(launch(ctx->fn_curv[metric->kind], ((nn + 7) / 8) * 64, 64, s, args));
@@ -355,10 +381,12 @@ int geon_nec_contraction_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const
     if (!ctx || !metric || !x_soa || !k || !out) return fail(GEON_E_INVALID, "bad argument");
     if (n == 0) return GEON_OK;
     hipSetDevice(ctx->device);
-    void* d_t;
-    GEON_TRY(ctx_buffer(ctx, "nec_t", n * 16 * 8, &d_t));
-    GEON_TRY(geon_curvature_dev(ctx, metric, x_soa, n, GEON_CURV_STRESS, (double*)d_t, stream));
     hipStream_t s = pick_stream(ctx, stream);
+    geon_lane* lane;
+    GEON_TRY(ctx_lane(ctx, s, &lane));
+    void* d_t;
+    GEON_TRY(lane_buffer(lane, "nec_t", n * 16 * 8, &d_t));
+    GEON_TRY(geon_curvature_dev(ctx, metric, x_soa, n, GEON_CURV_STRESS, (double*)d_t, s));
     uint64_t nn = n;
     double k0 = k[0], k1 = k[1], k2 = k[2], k3 = k[3];
     void* args[] = {&d_t, &nn, &k0, &k1, &k2, &k3, &out};

@@ -51,7 +51,8 @@ struct geon_batch {
     std::vector<geon_djob> h_tab[4][2];
     std::vector<unsigned long long> h_base[4][2];
     hipEvent_t ev_start = nullptr, ev_end = nullptr;
-    bool ran = false;
+    bool enqueued = false;   // some work of the run was enqueued
+    bool ran = false;        // the whole run was enqueued (ev_end marks its end)
 };
 
 namespace geon_host {
@@ -75,9 +76,8 @@ int frame_from(const geon_metric_t* metric, const geon_camera_t* cam, const geon
         if (cam->width == 0 || cam->height == 0) return fail(GEON_E_INVALID, "camera has zero size");
         GEON_TRY(tetrad(metric, cam->observer, F->tet));
         memcpy(F->obs, cam->observer, sizeof F->obs);
-        const double fov_rad = cam->fov_y_deg * (kPi / 180.0);
         const double aspect = (double)cam->width / (double)cam->height;
-        F->h_im = tan(fov_rad * 0.5);
+        F->h_im = tan(cam->fov_y_rad * 0.5);
         F->w_im = F->h_im * aspect;
         F->W = cam->width;
         F->H = cam->height;
@@ -141,6 +141,7 @@ static int sched_launch(geon_ctx* c, int kind, int f32, const geon_sched_cfg_t& 
 int batch_run_on(geon_batch* b, const geon_sched_cfg_t* tune, hipStream_t user) {
     geon_ctx* c = b->ctx;
     if (b->jobs.empty()) return GEON_OK;
+    b->enqueued = true;
     // ---- build the per-queue job tables ----
     auto& tab = b->h_tab;
     auto& base = b->h_base;
@@ -263,15 +264,24 @@ int geon_batch_create(geon_ctx_t* ctx, geon_batch_t** out) {
     return GEON_OK;
 }
 
-void geon_batch_destroy(geon_batch_t* b) {
-    if (!b) return;
-    hipSetDevice(b->ctx->device);
-    hipDeviceSynchronize();
+// Frees a batch whose enqueued work has completed.
+static void batch_free(geon_batch* b) {
     for (void* p : b->owned) hipFree(p);
     if (b->d_counters) hipFree(b->d_counters);
     if (b->ev_start) hipEventDestroy(b->ev_start);
     if (b->ev_end) hipEventDestroy(b->ev_end);
     delete b;
+}
+
+// Waits for the batch's own work only: a run that was enqueued whole ends at
+// ev_end; one that failed part-way has no end marker, so the device is
+// drained before its buffers go.
+void geon_batch_destroy(geon_batch_t* b) {
+    if (!b) return;
+    hipSetDevice(b->ctx->device);
+    if (b->ran) hipEventSynchronize(b->ev_end);
+    else if (b->enqueued) hipDeviceSynchronize();
+    batch_free(b);
 }
 
 static int add_job(geon_batch_t* b, HostJob& j) {
@@ -347,14 +357,14 @@ int geon_batch_run(geon_batch_t* b, const geon_sched_cfg_t* tune, void* stream) 
     if (!b) return fail(GEON_E_INVALID, "bad argument");
     if (b->ran) return fail(GEON_E_INVALID, "batch already ran");
     hipSetDevice(b->ctx->device);
-    // counters: per-kind block, then per job: stats (8 words), done, fp64
+    // counters: per-kind block, then per job: stats, done, fp64
     const size_t per_kind = 4 * C_PER_KIND;
     size_t w = per_kind;
     for (auto& j : b->jobs) {
         j.stats_word = w;
-        j.done_word = w + 8;
-        j.fp64_word = w + 9;
-        w += 10;
+        j.done_word = w + GEON_DSTATS_WORDS;
+        j.fp64_word = w + GEON_DSTATS_WORDS + 1;
+        w += GEON_DSTATS_WORDS + 2;
     }
     b->n_counter_words = w;
     hipError_t e = hipMalloc((void**)&b->d_counters, w * 8);
@@ -373,7 +383,7 @@ int geon_batch_frame_stats(geon_batch_t* b, uint32_t frame_id, geon_render_stats
     if (!b->ran) return fail(GEON_E_INVALID, "batch has not run");
     const HostJob& j = b->jobs[frame_id];
     GEON_HIP(hipEventSynchronize(b->ev_end), "sync");
-    unsigned long long w[10];
+    unsigned long long w[GEON_DSTATS_WORDS + 2];
     GEON_HIP(hipMemcpyDtoH(w, (hipDeviceptr_t)(b->d_counters + j.stats_word), sizeof w), "D2H");
     geon_dstats ds;
     memcpy(&ds, w, sizeof ds);
@@ -388,7 +398,7 @@ int geon_batch_frame_stats(geon_batch_t* b, uint32_t frame_id, geon_render_stats
     out->avg_steps = n_pix ? (double)ds.steps_pix_sum / (double)n_pix : 0.0;
     out->max_steps_hit = ds.max_steps_hit;
     out->ray_steps = ds.ray_steps;
-    out->fp64_rays = j.dj.F.precision == GEON_PREC_MIXED ? w[9] : j.n_rays;
+    out->fp64_rays = j.dj.F.precision == GEON_PREC_MIXED ? w[GEON_DSTATS_WORDS + 1] : j.n_rays;
     float ms = 0.0f;
     hipEventElapsedTime(&ms, b->ev_start, b->ev_end);
     out->kernel_ms = ms;
@@ -422,6 +432,21 @@ int geon_batch_totals(geon_batch_t* b, uint64_t* ray_steps, uint64_t* fp64_rays,
 }  // extern "C"
 #pragma GCC visibility pop
 
+namespace geon_host {
+
+// Batches handed to a lane by a device-pointer render: freed once their
+// end event has passed (all of them, waiting, when `wait`).
+void lane_reap(geon_lane* l, bool wait) {
+    std::vector<geon_batch*> keep;
+    for (geon_batch* b : l->retired) {
+        if (wait || !b->ran || hipEventQuery(b->ev_end) == hipSuccess) geon_batch_destroy(b);
+        else keep.push_back(b);
+    }
+    l->retired.swap(keep);
+}
+
+}  // namespace geon_host
+
 // ---- autotuner ------------------------------------------------------------------
 namespace {
 
@@ -438,7 +463,7 @@ void calibration(int kind, geon_metric_t* m, geon_camera_t* cam, geon_render_cfg
     geon_metric_default(kind, m);
     memset(cam, 0, sizeof *cam);
     memset(cfg, 0, sizeof *cfg);
-    cam->fov_y_deg = 60.0;
+    cam->fov_y_rad = 60.0 * (kPi / 180.0);
     cam->width = cam->height = 2048;
     cfg->step = 0.05;
     cfg->max_steps = 1000;

@@ -255,23 +255,33 @@ int ctx_create(int device, const char* code_object, geon_ctx** out) {
         c->tuned[k][0] = {256, 4, 2, 64};
         c->tuned[k][1] = {256, 4, 2, 64};
     }
-    hipEventCreate(&c->t0);
-    hipEventCreate(&c->t1);
-    if ((e = hipMalloc((void**)&c->d_stats, sizeof(geon_dstats))) != hipSuccess) {
+    geon_lane* l0;
+    const int st = ctx_lane(c, c->stream, &l0);
+    if (st != GEON_OK) {
         ctx_destroy(c);
-        return fail(GEON_E_NOMEM, "stats block: %s", hipGetErrorString(e));
+        return st;
     }
     *out = c;
     return GEON_OK;
+}
+
+// The lane's retired batches are freed first by the caller (lane_reap).
+static void lane_free(geon_lane* l) {
+    if (l->stream) hipStreamSynchronize(l->stream);
+    for (auto& kv : l->bufs)
+        if (kv.second.p) hipFree(kv.second.p);
+    if (l->d_stats) hipFree(l->d_stats);
+    if (l->t0) hipEventDestroy(l->t0);
+    if (l->t1) hipEventDestroy(l->t1);
+    delete l;
 }
 
 void ctx_destroy(geon_ctx* c) {
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
-    for (auto& kv : c->bufs)
-        if (kv.second.p) hipFree(kv.second.p);
-    if (c->d_stats) hipFree(c->d_stats);
+    for (auto& kv : c->lanes) lane_free(kv.second);
+    c->lanes.clear();
     for (int s = 0; s < GEON_CONST_SLOTS; ++s)
         if (c->slot_ev[s]) hipEventDestroy(c->slot_ev[s]);
     for (int k = 0; k < 4; ++k) {
@@ -279,18 +289,41 @@ void ctx_destroy(geon_ctx* c) {
         if (c->join_ev[k]) hipEventDestroy(c->join_ev[k]);
     }
     if (c->fork_ev) hipEventDestroy(c->fork_ev);
-    if (c->t0) hipEventDestroy(c->t0);
-    if (c->t1) hipEventDestroy(c->t1);
     if (c->stream) hipStreamDestroy(c->stream);
     if (c->mod) hipModuleUnload(c->mod);
     delete c;
 }
 
-int ctx_buffer(geon_ctx* c, const char* name, size_t bytes, void** p) {
-    geon_ctx::Buf& b = c->bufs[name];
+int ctx_lane(geon_ctx* c, hipStream_t stream, geon_lane** out) {
+    if (!stream) stream = c->stream;
+    auto it = c->lanes.find(stream);
+    if (it != c->lanes.end()) {
+        *out = it->second;
+        return GEON_OK;
+    }
+    geon_lane* l = new geon_lane();
+    l->stream = stream;
+    hipError_t e = hipMalloc((void**)&l->d_stats, sizeof(geon_dstats));
+    if (e != hipSuccess) {
+        delete l;
+        return fail(GEON_E_NOMEM, "stats block: %s", hipGetErrorString(e));
+    }
+    if ((e = hipEventCreate(&l->t0)) != hipSuccess || (e = hipEventCreate(&l->t1)) != hipSuccess) {
+        l->stream = nullptr;   // nothing was enqueued on it yet
+        lane_free(l);
+        return hip_fail(e, "hipEventCreate");
+    }
+    c->lanes[stream] = l;
+    *out = l;
+    return GEON_OK;
+}
+
+int lane_buffer(geon_lane* l, const char* name, size_t bytes, void** p) {
+    geon_lane::Buf& b = l->bufs[name];
     if (b.bytes < bytes) {
         if (b.p) {
-            hipStreamSynchronize(c->stream);
+            // the lane's earlier work may still read or write the old buffer
+            hipStreamSynchronize(l->stream);
             hipFree(b.p);
             b.p = nullptr;
             b.bytes = 0;
@@ -302,6 +335,12 @@ int ctx_buffer(geon_ctx* c, const char* name, size_t bytes, void** p) {
     }
     *p = b.p;
     return GEON_OK;
+}
+
+int ctx_buffer(geon_ctx* c, const char* name, size_t bytes, void** p) {
+    geon_lane* l;
+    GEON_TRY(ctx_lane(c, c->stream, &l));
+    return lane_buffer(l, name, bytes, p);
 }
 
 // Claim the next constant slot and enqueue its upload on `stream`. A slot is

@@ -4,9 +4,28 @@
 
 #include <map>
 #include <string>
+#include <vector>
 
 #include "../../../include/geon_hip.h"
 #include "../geon_dev_types.h"
+
+struct geon_batch;
+
+// Scratch the context keeps for the work enqueued on one stream: device
+// buffers by name (grow-only), a statistics block and two timing events.
+// Work on different streams never shares a lane, so "_dev" calls on
+// different streams may overlap on the device.
+struct geon_lane {
+    hipStream_t stream = nullptr;
+    struct Buf {
+        void* p = nullptr;
+        size_t bytes = 0;
+    };
+    std::map<std::string, Buf> bufs;
+    geon_dstats* d_stats = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    std::vector<geon_batch*> retired;   // batches whose work may still run on `stream`
+};
 
 struct geon_ctx {
     int device = 0;
@@ -34,13 +53,7 @@ struct geon_ctx {
     hipEvent_t slot_ev[GEON_CONST_SLOTS] = {};
     bool slot_used[GEON_CONST_SLOTS] = {};
     uint32_t next_slot = 0;
-    hipEvent_t t0 = nullptr, t1 = nullptr;
-    geon_dstats* d_stats = nullptr;
-    struct Buf {
-        void* p = nullptr;
-        size_t bytes = 0;
-    };
-    std::map<std::string, Buf> bufs;
+    std::map<hipStream_t, geon_lane*> lanes;   // per stream handle; the context stream's too
 };
 
 namespace geon_host {
@@ -56,7 +69,15 @@ int make_dmetric(const geon_metric_t* m, geon_dmetric* d);
 
 int ctx_create(int device, const char* code_object, geon_ctx** out);
 void ctx_destroy(geon_ctx* c);
+// The lane of `stream` (NULL: the context stream), created on first use.
+int ctx_lane(geon_ctx* c, hipStream_t stream, geon_lane** out);
+// A device buffer of at least `bytes` owned by the lane; growing it waits
+// for the lane's stream before the old buffer is freed.
+int lane_buffer(geon_lane* l, const char* name, size_t bytes, void** p);
+// The context stream's lane buffer (host-buffer entry points).
 int ctx_buffer(geon_ctx* c, const char* name, size_t bytes, void** p);
+// Frees the lane's retired batches whose work has completed (all when wait).
+void lane_reap(geon_lane* l, bool wait);
 int ctx_slot(geon_ctx* c, const geon_dframe* f, hipStream_t stream, uint32_t* slot);
 int ctx_slot_done(geon_ctx* c, uint32_t slot, hipStream_t stream);
 int launch(hipFunction_t f, uint64_t n_threads, uint32_t block, hipStream_t stream, void** args);

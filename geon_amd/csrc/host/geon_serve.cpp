@@ -39,7 +39,7 @@ using namespace geon_host;
 namespace {
 
 const double kHalfPi = 1.5707963267948966;
-const int kJobWords = 10;  // per job: done, fp64, geon_dstats (8)
+const int kJobWords = 2 + (int)GEON_DSTATS_WORDS;  // per job: done, fp64, geon_dstats
 const int kNoRoom = 1;     // internal: the arena cannot hold the request yet
 
 double now_ms() {

@@ -433,6 +433,10 @@ struct Minkowski {
 struct Schwarzschild {
     static constexpr int flat_test = 0;
     static constexpr bool g_nz(int m, int n) { return m == n; }
+    // g depends on r (every diagonal entry) and θ (g_φφ)
+    static constexpr bool dg_nz(int a, int m, int n) {
+        return (a == 1 && m == n) || (a == 2 && m == 3 && n == 3);
+    }
     static constexpr bool gam_nz(int mu, int a, int b) {
         return (mu == 0 && ((a == 0 && b == 1) || (a == 1 && b == 0))) ||
                (mu == 1 && a == b) ||
@@ -839,16 +843,18 @@ GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, doub
     for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
 }
 
+// Returns true when the step evaluated Γ at its four stages, false when it
+// was a flat step.
 template <class MP>
-GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* out, FlatRun& fr) {
+GEON_D bool rk4_step(const geon_dmetric& P, const double* y, double h, double* out, FlatRun& fr) {
     if constexpr (MP::flat_test == 1) {
         rk4_step_flat(y, h, out);
-        return;
+        return false;
     } else if constexpr (MP::flat_test == 2) {
         if (fr.left > 0) {
             --fr.left;
             rk4_step_flat(y, h, out);
-            return;
+            return false;
         }
         if (fr.nf) {
             fr.nf = false;
@@ -857,11 +863,12 @@ GEON_D void rk4_step(const geon_dmetric& P, const double* y, double h, double* o
             if (k > 0) {
                 fr.left = k - 1;
                 rk4_step_flat(y, h, out);
-                return;
+                return false;
             }
         }
     }
     rk4_step_full<MP>(P, y, h, out);
+    return true;
 }
 
 template <class MP>
@@ -987,15 +994,19 @@ GEON_D uint32_t flat_span(const geon_dmetric& P, double* y, double h, FlatRun& f
 
 // trace (geodesic.rs:62-117): terminate first, step, NaN guard keeps the
 // pre-step state, max_steps → MAXSTEPS. Runs of flat steps go as spans.
+// gamma_steps (optional) receives the number of steps that evaluated Γ
+// (every step of a metric without flat regions).
 template <class MP>
-GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps, uint32_t* steps) {
+GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps, uint32_t* steps,
+                 uint32_t* gamma_steps = nullptr) {
     FlatRun fr;
-    uint32_t n = 0;
+    uint32_t n = 0, ng = 0;
+    int cls = MAXSTEPS;
     while (n < max_steps) {
         const int t = MP::terminate(P, y);
         if (t != RUNNING) {
-            *steps = n;
-            return t;
+            cls = t;
+            break;
         }
         if constexpr (MP::flat_test != 0) {
             const uint32_t m = flat_span<MP>(P, y, h, fr, max_steps - n);
@@ -1005,17 +1016,23 @@ GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps,
             }
         }
         double yn[8];
-        rk4_step<MP>(P, y, h, yn, fr);
+        const bool full = rk4_step<MP>(P, y, h, yn, fr);
         if (!finite8(yn)) {
-            *steps = n;
-            return DIVERGED;
+            cls = DIVERGED;
+            break;
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) y[i] = yn[i];
         ++n;
+        if constexpr (MP::flat_test == 2) ng += full ? 1u : 0u;
     }
-    *steps = max_steps;
-    return MAXSTEPS;
+    *steps = n;
+    if (gamma_steps) {
+        if constexpr (MP::flat_test == 0) *gamma_steps = n;
+        else if constexpr (MP::flat_test == 1) *gamma_steps = 0;
+        else *gamma_steps = ng;
+    }
+    return cls;
 }
 
 // ---- adaptive stepping -------------------------------------------------------

@@ -16,34 +16,45 @@ extern "C" {
 __constant__ geon_dframe c_frames[GEON_CONST_SLOTS];
 }
 
+// Σ over the wave's lanes of v, added to *p by one lane (every lane of the
+// wave must be here).
+__device__ void wave_add_u64(unsigned long long* p, unsigned long long v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    if (__lane_id() == 0 && v != 0ull) atomicAdd(p, v);
+}
+
 // ---- render: one lane per (pixel, sub-sample) ray -------------------------
 // ray r → pixel r / spa², sub-sample r % spa² (render.rs:89-92), so the
 // sub-rays of one pixel share a wave and their trajectories stay coherent.
+// stats->gamma_steps receives the launch's steps that evaluated Γ.
 template <class MP>
 __device__ void render_rays(uint32_t slot, uint64_t n_rays, int32_t* __restrict__ cls,
                             uint32_t* __restrict__ steps, double* __restrict__ resid,
-                            double* __restrict__ dir) {
+                            double* __restrict__ dir, geon_dstats* __restrict__ stats) {
     const geon_dframe& F = c_frames[slot];
     const uint64_t ray = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (ray >= n_rays) return;
-    const uint32_t spa = F.spa, spa2 = spa * spa;
-    const uint64_t pix = ray / spa2;
-    const uint32_t sub = (uint32_t)(ray - pix * spa2);
-    const uint32_t i = (uint32_t)(pix % F.W), j = (uint32_t)(pix / F.W);
-    const double du = ((double)(sub % spa) + 0.5) * F.inv_spa;
-    const double dv = ((double)(sub / spa) + 0.5) * F.inv_spa;
-    double y[8];
-    camera_ray(F, i, j, du, dv, y);
-    uint32_t n;
-    const int c = trace<MP>(F.m, y, F.step, F.max_steps, &n);
-    cls[ray] = c;
-    steps[ray] = n;
-    resid[ray] = null_residual<MP>(F.m, y);
-    double d[3] = {0.0, 0.0, 0.0};
-    if (c == ESCAPED) direction<MP>(y, d);
-    dir[ray] = d[0];
-    dir[n_rays + ray] = d[1];
-    dir[2 * n_rays + ray] = d[2];
+    uint32_t ng = 0;
+    if (ray < n_rays) {
+        const uint32_t spa = F.spa, spa2 = spa * spa;
+        const uint64_t pix = ray / spa2;
+        const uint32_t sub = (uint32_t)(ray - pix * spa2);
+        const uint32_t i = (uint32_t)(pix % F.W), j = (uint32_t)(pix / F.W);
+        const double du = ((double)(sub % spa) + 0.5) * F.inv_spa;
+        const double dv = ((double)(sub / spa) + 0.5) * F.inv_spa;
+        double y[8];
+        camera_ray(F, i, j, du, dv, y);
+        uint32_t n;
+        const int c = trace<MP>(F.m, y, F.step, F.max_steps, &n, &ng);
+        cls[ray] = c;
+        steps[ray] = n;
+        resid[ray] = null_residual<MP>(F.m, y);
+        double d[3] = {0.0, 0.0, 0.0};
+        if (c == ESCAPED) direction<MP>(y, d);
+        dir[ray] = d[0];
+        dir[n_rays + ray] = d[1];
+        dir[2 * n_rays + ray] = d[2];
+    }
+    if (stats) wave_add_u64(&stats->gamma_steps, ng);
 }
 
 // ---- trace from caller-supplied initial states ----------------------------
@@ -113,8 +124,8 @@ __device__ void trace_states_adaptive(uint32_t slot, uint64_t n, const double* _
 #define GEON_KERNELS(NAME, MP, WAVES)                                                           \
     extern "C" __global__ __launch_bounds__(256, WAVES) void geon_render_rays_##NAME(           \
         uint32_t slot, uint64_t n_rays, int32_t* cls, uint32_t* steps, double* resid,          \
-        double* dir) {                                                                          \
-        render_rays<MP>(slot, n_rays, cls, steps, resid, dir);                                  \
+        double* dir, geon_dstats* stats) {                                                      \
+        render_rays<MP>(slot, n_rays, cls, steps, resid, dir, stats);                           \
     }                                                                                           \
     extern "C" __global__ __launch_bounds__(256, WAVES) void geon_trace_states_##NAME(          \
         uint32_t slot, uint64_t n, const double* y0, int32_t* cls, uint32_t* steps,            \
@@ -285,7 +296,7 @@ extern "C" __global__ void geon_stats_clear(geon_dstats* stats) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         stats->captured = stats->escaped = stats->diverged = 0;
         stats->max_resid_bits = stats->steps_pix_sum = stats->max_steps_hit = 0;
-        stats->ray_steps = stats->fp64_rays = 0;
+        stats->ray_steps = stats->fp64_rays = stats->gamma_steps = 0;
     }
 }
 

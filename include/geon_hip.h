@@ -12,7 +12,12 @@
  *                             src/metric/alcubierre.rs:21-30,
  *                             src/metric/morris_thorne.rs:14-19,
  *                             src/metric/minkowski.rs:8)
+ *   geon_metric_{g,dg,g_inv,christoffel,terminate,asymptotic_direction}
+ *                            the Metric trait's methods on the host
+ *                            (src/metric/mod.rs:19-109)
+ *   geon_camera_new          Camera::new               (src/camera.rs:29-36)
  *   geon_tetrad              Camera::tetrad            (src/camera.rs:47-97)
+ *   geon_ray_state           Camera::ray_state_sub     (src/camera.rs:113-160)
  *   geon_trace_rays[_dev]    geodesic::trace, batched  (src/geodesic.rs:62-117)
  *   geon_trace_rays_adaptive[_dev] no counterpart: the adaptive stepping the
  *                            reference names as its next step
@@ -28,9 +33,13 @@
  *   geon_tonemap_u8          render::write_png tonemap (src/render.rs:169-180)
  *
  * Memory: "_dev" entry points take device pointers (HBM) and a hipStream_t
- * passed as void*; they enqueue work and return without synchronising.
- * The other entry points take caller-owned host buffers, copy in and out,
- * and synchronise. The context owns every device allocation it makes.
+ * passed as void* (NULL: the context's own stream); they enqueue work and
+ * return without synchronising the host, except where a host output
+ * (statistics) is requested, which waits for that stream only. Scratch
+ * memory the context keeps for them is held per stream, so calls on
+ * different streams may overlap on the device. The other entry points take
+ * caller-owned host buffers, copy in and out, and synchronise. The context
+ * owns every device allocation it makes.
  *
  * Threading: a context is bound to one device and used by one host thread
  * at a time. Separate contexts may be used concurrently.
@@ -87,10 +96,12 @@ typedef struct geon_metric {
     double  fd_step;
 } geon_metric_t;
 
-/* Camera::new(observer, fov_y_deg, width, height)  (camera.rs:29-36) */
+/* Camera (camera.rs:21-26): the vertical field of view in radians, as the
+ * reference stores it. Camera::new(observer, fov_y_deg, width, height)
+ * (camera.rs:29-36) is geon_camera_new. */
 typedef struct geon_camera {
     double   observer[4];
-    double   fov_y_deg;
+    double   fov_y_rad;
     uint32_t width;
     uint32_t height;
 } geon_camera_t;
@@ -124,6 +135,11 @@ typedef struct geon_render_stats {
     uint64_t ray_steps;         /* sum of steps_taken over every traced ray   */
     uint64_t fp64_rays;         /* rays integrated in FP64 (mixed precision)  */
     double   kernel_ms;         /* device time of the integration kernels     */
+    uint64_t gamma_steps;       /* of ray_steps, the steps that evaluated Γ at
+                                 * their stages (the rest are steps through
+                                 * regions where Γ is exactly zero: Minkowski,
+                                 * Alcubierre beyond its walls); plain FP64
+                                 * renders only, 0 elsewhere                  */
 } geon_render_stats_t;
 
 /* Per-ray outcome codes. The reference's Termination has three variants;
@@ -148,6 +164,33 @@ const char* geon_last_error(void);
 int  geon_version(void);
 
 int  geon_metric_default(int32_t kind, geon_metric_t* out);
+
+/* ---- the Metric trait on the host (src/metric/mod.rs:19-109) ------------
+ * Host evaluations of one metric at one point, so that a binding can
+ * implement the reference's trait for the metrics this library runs on the
+ * GPU. Matrices are row-major; dg is [α][μ][ν], Γ is [μ][α][β]. Each follows
+ * the reference's definition for that metric: analytic g⁻¹ and Γ for
+ * Schwarzschild (schwarzschild.rs:47-107), zero ∂g and Γ for Minkowski
+ * (minkowski.rs:20-30), and the trait defaults elsewhere: central
+ * differences of g with fd_step (mod.rs:25-43), the cofactor inverse
+ * (mod.rs:46-48, types.rs:34-46; GEON_E_SINGULAR when g is not invertible)
+ * and Γ = ½ g^μσ(∂_α g_σβ + ∂_β g_σα − ∂_σ g_αβ) (mod.rs:51-69). */
+int  geon_metric_g(const geon_metric_t* metric, const double x[4], double g_out[16]);
+int  geon_metric_dg(const geon_metric_t* metric, const double x[4], double dg_out[64]);
+int  geon_metric_g_inv(const geon_metric_t* metric, const double x[4], double ginv_out[16]);
+int  geon_metric_christoffel(const geon_metric_t* metric, const double x[4], double gamma_out[64]);
+/* Metric::terminate (mod.rs:78-81 and each metric's override): *cls_out is
+ * GEON_CAPTURED, GEON_ESCAPED, GEON_DIVERGED, or -1 (None: keep integrating). */
+int  geon_metric_terminate(const geon_metric_t* metric, const double x[4], int32_t* cls_out);
+/* Metric::asymptotic_direction (mod.rs:91-108; alcubierre.rs:81-88;
+ * morris_thorne.rs:54-77): unit Cartesian direction of an escaped ray. */
+int  geon_metric_asymptotic_direction(const geon_metric_t* metric, const double x[4],
+                                      const double p[4], double dir_out[3]);
+
+/* Camera::new (camera.rs:29-36): fov_y_rad = fov_y_deg·(π/180), as
+ * f64::to_radians computes it. */
+int  geon_camera_new(const double observer[4], double fov_y_deg, uint32_t width,
+                     uint32_t height, geon_camera_t* out);
 
 /* code_object: path of geon_kernels.hsaco, or NULL for the copy installed
  * beside libgeon_hip.so. */

@@ -618,8 +618,16 @@ int or_tetrad(const geon_metric_t* m, const double observer[4], double tet[16]) 
     return GEON_OK;
 }
 
-/* Camera::new: fov_y_deg.to_radians() = deg * (PI/180) (camera.rs:29-36). */
-static double fov_rad(const geon_camera_t* cam) { return cam->fov_y_deg * (OR_PI / 180.0); }
+/* Camera::new: fov_y_rad = fov_y_deg.to_radians() = deg * (PI/180)
+ * (camera.rs:29-36); Camera holds the radians (camera.rs:21-26). */
+void or_camera_new(const double observer[4], double fov_y_deg, uint32_t width, uint32_t height,
+                   geon_camera_t* out) {
+    memcpy(out->observer, observer, sizeof out->observer);
+    out->fov_y_rad = fov_y_deg * (OR_PI / 180.0);
+    out->width = width;
+    out->height = height;
+}
+static double fov_rad(const geon_camera_t* cam) { return cam->fov_y_rad; }
 
 /* camera.rs:113-160 — pixel (i,j)+(du,dv) → local dir (−1, −v, u)/‖·‖ →
  * p^μ = e0 + n1 e1 + n2 e2 + n3 e3. */

@@ -61,6 +61,8 @@ double or_trace_rays(const geon_metric_t* m, const double* y0_aos, size_t n,
                      double* resid, double* dir_aos);
 
 /* L3 camera */
+void   or_camera_new(const double observer[4], double fov_y_deg, uint32_t width,
+                     uint32_t height, geon_camera_t* out);
 int    or_tetrad(const geon_metric_t* m, const double observer[4], double tet[16]);
 void   or_ray_state_sub(const geon_camera_t* cam, const double tet[16], uint64_t i,
                         uint64_t j, double du, double dv, double y[8]);

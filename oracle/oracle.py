@@ -30,7 +30,7 @@ class Metric(C.Structure):
 
 
 class Camera(C.Structure):
-    _fields_ = [("observer", C.c_double * 4), ("fov_y_deg", C.c_double),
+    _fields_ = [("observer", C.c_double * 4), ("fov_y_rad", C.c_double),
                 ("width", C.c_uint32), ("height", C.c_uint32)]
 
 
@@ -45,7 +45,8 @@ class RenderStats(C.Structure):
     _fields_ = [("captured", C.c_uint64), ("escaped", C.c_uint64), ("diverged", C.c_uint64),
                 ("max_null_residual", C.c_double), ("avg_steps", C.c_double),
                 ("max_steps_hit", C.c_uint64), ("ray_steps", C.c_uint64),
-                ("fp64_rays", C.c_uint64), ("kernel_ms", C.c_double)]
+                ("fp64_rays", C.c_uint64), ("kernel_ms", C.c_double),
+                ("gamma_steps", C.c_uint64)]
 
 
 class TraceResult(C.Structure):
@@ -117,10 +118,11 @@ def metric(kind: int, **kw) -> Metric:
 
 
 def camera(observer, fov_y_deg, width, height) -> Camera:
+    """Camera::new (camera.rs:29-36): the field of view in degrees, held in radians."""
     c = Camera()
-    for i in range(4):
-        c.observer[i] = float(observer[i])
-    c.fov_y_deg, c.width, c.height = float(fov_y_deg), int(width), int(height)
+    obs = np.ascontiguousarray(observer, dtype=np.float64)
+    lib().or_camera_new(_dp(obs), C.c_double(fov_y_deg), C.c_uint32(int(width)),
+                        C.c_uint32(int(height)), C.byref(c))
     return c
 
 

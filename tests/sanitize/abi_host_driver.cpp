@@ -23,10 +23,17 @@ int main() {
         double tet[16];
         bad += expect(geon_tetrad(&m, obs, tet), GEON_OK, "tetrad");
         geon_camera_t cam;
-        memcpy(cam.observer, obs, sizeof obs);
-        cam.fov_y_deg = 60.0;
-        cam.width = 7;
-        cam.height = 5;
+        bad += expect(geon_camera_new(obs, 60.0, 7, 5, &cam), GEON_OK, "camera_new");
+        // the Metric trait on the host, at the observer
+        double g[16], dg[64], gi[16], gam[64], dir[3];
+        int32_t cls;
+        bad += expect(geon_metric_g(&m, obs, g), GEON_OK, "metric_g");
+        bad += expect(geon_metric_dg(&m, obs, dg), GEON_OK, "metric_dg");
+        bad += expect(geon_metric_g_inv(&m, obs, gi), GEON_OK, "metric_g_inv");
+        bad += expect(geon_metric_christoffel(&m, obs, gam), GEON_OK, "metric_christoffel");
+        bad += expect(geon_metric_terminate(&m, obs, &cls), GEON_OK, "metric_terminate");
+        bad += expect(geon_metric_asymptotic_direction(&m, obs, tet, dir), GEON_OK, "direction");
+        for (int c = 0; c < 64; ++c) bad += isfinite(gam[c]) ? 0 : 1;
         for (uint32_t j = 0; j < cam.height; ++j)
             for (uint32_t i = 0; i < cam.width; ++i) {
                 double y[8];

@@ -29,7 +29,8 @@ def tmp(tmp_path_factory):
 
 def test_host_runtime_under_sanitizers(tmp):
     host = [os.path.join(ROOT, "geon_amd", "csrc", "host", f)
-            for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp", "geon_serve.cpp")]
+            for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp", "geon_serve.cpp",
+                      "geon_metric_host.cpp")]
     exe = str(tmp / "abi_driver")
     b = _run(["g++", "-std=c++17", *SAN, "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
               f"-I{os.path.join(ROOT, 'include')}",
