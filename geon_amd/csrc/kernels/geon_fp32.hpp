@@ -5,6 +5,10 @@
 // difference with h = 1e-5 is pure rounding noise in FP32, so the generic
 // FD ∂g of the reference is replaced by the analytic ∂g it approximates
 // (its O(h²) truncation, ~1e-10 relative, is far below FP32 resolution).
+// Every fused multiply-add is written out (fmaf) and nothing else is
+// contracted: the compiler is free to fuse differently in each instantiation
+// and unrolled copy of a step, which would make a ray's rounding depend on
+// the kernel and slot it runs in; written out, it is fixed by the source.
 // After each step p^t is re-projected onto the null cone g(p,p) = 0
 // (future-directed root), which keeps FP32 drift out of the
 // energy-carrying component. Where Γ vanishes at every stage (Minkowski,
@@ -73,6 +77,10 @@ GEON_D bool pole_pass(const FParams& P, float min_s, float min_r) {
 
 GEON_D float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 GEON_D float fsqrt(float x) { return __builtin_sqrtf(x); }
+// a² + b² + c² as fmaf(a, a, fmaf(b, b, c·c))
+GEON_D float norm3sq(float a, float b, float c) {
+    return __builtin_fmaf(a, a, __builtin_fmaf(b, b, c * c));
+}
 
 // sin x and cos x in FP32 for |x| < 2^13 (else the library sincosf): the
 // Cody–Waite reduction x = k·π/2 + r with π/2 in three floats (each product
@@ -80,7 +88,6 @@ GEON_D float fsqrt(float x) { return __builtin_sqrtf(x); }
 // on |r| ≤ π/4 (truncation below 1e-9), and the quadrant k mod 4.
 #ifndef GEON_OPCOUNT   // the op-counting build tallies this as one sine and one cosine
 GEON_D void sincosf_cw(float x, float* s, float* c) {
-#pragma clang fp contract(fast)
     if (!(fabsf(x) < 8192.0f)) {
         sincosf(x, s, c);
         return;
@@ -108,9 +115,8 @@ GEON_D void sincosf_cw(float x, float* s, float* c) {
 
 // tanh via one exp: tanh(u) = 1 − 2/(e^{2u} + 1), exact saturation at ±1
 GEON_D float ftanh(float u) {
-#pragma clang fp contract(fast)
     const float e = __expf(2.0f * u);
-    return 1.0f - 2.0f * frcp(e + 1.0f);
+    return __builtin_fmaf(-2.0f, frcp(e + 1.0f), 1.0f);
 }
 
 // ---- Minkowski --------------------------------------------------------------
@@ -120,12 +126,10 @@ struct MinkowskiF {
         dp[0] = dp[1] = dp[2] = dp[3] = 0.0f;
     }
     GEON_D static void reproject(const FParams&, float* y) {
-#pragma clang fp contract(fast)
-        y[4] = fsqrt(y[5] * y[5] + y[6] * y[6] + y[7] * y[7]);
+        y[4] = fsqrt(norm3sq(y[5], y[6], y[7]));
     }
     GEON_D static int terminate(const FParams& P, const float* x) {
-#pragma clang fp contract(fast)
-        const float r2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+        const float r2 = norm3sq(x[1], x[2], x[3]);
         return (r2 > P.esc2) ? ESCAPED : RUNNING;
     }
     GEON_D static bool suspicious(const FParams&, int cls, const float*, float, float) {
@@ -142,33 +146,34 @@ struct SchwarzschildF {
     // 1/r, 1/(r − 2M) and cot θ from one reciprocal of r·(r − 2M)·sin θ
     // (reciprocals issue at a quarter of the VALU rate)
     GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
-#pragma clang fp contract(fast)
         const float r = y[1];
         const float pt = y[4], pr = y[5], pth = y[6], pph = y[7];
         const bool pole = fabsf(s) < 1e-8f;
         const float se = pole ? 1.0f : s;
-        const float rf = r - 2.0f * P.M;             // r − 2M = r·f
+        const float rf = __builtin_fmaf(-2.0f, P.M, r);   // r − 2M = r·f
         const float rr = r * rf;
         const float q = frcp(rr * se);
-        const float inv_r = rf * se * q;
+        const float inv_r = (rf * se) * q;
         const float inv_rr = se * q;                 // 1/(r(r − 2M))
         const float f = rf * inv_r;
-        const float a = P.M * inv_r * inv_r;         // M/r²
+        const float a = (P.M * inv_r) * inv_r;       // M/r²
         const float gtr = P.M * inv_rr;              // Γ^t_tr = M/(r²f)
         const float grtt = a * f;                    // Γ^r_tt
-        const float cot = pole ? 0.0f : c * rr * q;
+        const float cot = pole ? 0.0f : (c * rr) * q;
         const float pph2 = pph * pph;
-        dp[0] = -2.0f * gtr * pt * pr;
-        dp[1] = -(grtt * pt * pt - gtr * pr * pr) + rf * (pth * pth + s * s * pph2);
-        dp[2] = -2.0f * inv_r * pr * pth + s * c * pph2;
-        dp[3] = -2.0f * pph * (inv_r * pr + cot * pth);
+        dp[0] = ((-2.0f * gtr) * pt) * pr;
+        // −(Γ^r_tt pt² − M/(r²f)·pr²) + (r − 2M)(pθ² + sin²θ pφ²)
+        dp[1] = __builtin_fmaf(rf, __builtin_fmaf(pth, pth, (s * s) * pph2),
+                               __builtin_fmaf(-grtt * pt, pt, (gtr * pr) * pr));
+        dp[2] = __builtin_fmaf((-2.0f * inv_r) * pr, pth, (s * c) * pph2);
+        dp[3] = (-2.0f * pph) * __builtin_fmaf(inv_r, pr, cot * pth);
     }
     GEON_D static void reproject(const FParams& P, float* y, float s) {
-#pragma clang fp contract(fast)
         const float r = y[1];
-        const float rf = r - 2.0f * P.M;
-        const float inv_f = r * r * frcp(r * rf);    // 1/f = r/(r − 2M)
-        const float sp = y[5] * y[5] * inv_f + r * r * (y[6] * y[6] + s * s * y[7] * y[7]);
+        const float rf = __builtin_fmaf(-2.0f, P.M, r);
+        const float inv_f = (r * r) * frcp(r * rf);  // 1/f = r/(r − 2M)
+        const float ang = __builtin_fmaf(y[6], y[6], (s * s) * (y[7] * y[7]));
+        const float sp = __builtin_fmaf(y[5] * y[5], inv_f, (r * r) * ang);
         if (rf > 0.0f && sp >= 0.0f) y[4] = fsqrt(sp * inv_f);
     }
     GEON_D static int terminate(const FParams& P, const float* x) {
@@ -194,26 +199,26 @@ struct MorrisThorneF {
     static constexpr bool uses_theta = true;
     static constexpr int flat_test = 0;
     GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
-#pragma clang fp contract(fast)
         const float l = y[1];
         const float pl = y[5], pth = y[6], pph = y[7];
         const bool pole = fabsf(s) < 1e-8f;
         const float se = pole ? 1.0f : s;
-        const float r2 = P.b0sq + l * l;
+        const float r2 = __builtin_fmaf(l, l, P.b0sq);
         const float q = frcp(r2 * se);               // 1/r² and cot θ from one reciprocal
         const float inv_r2 = se * q;
         const float lr = l * inv_r2;                 // Γ^θ_lθ = Γ^φ_lφ
-        const float cot = pole ? 0.0f : c * r2 * q;
+        const float cot = pole ? 0.0f : (c * r2) * q;
         const float pph2 = pph * pph;
         dp[0] = 0.0f;
-        dp[1] = l * (pth * pth + s * s * pph2);      // −(Γ^l_θθ pθ² + Γ^l_φφ pφ²)
-        dp[2] = -2.0f * lr * pl * pth + s * c * pph2;
-        dp[3] = -2.0f * pph * (lr * pl + cot * pth);
+        // −(Γ^l_θθ pθ² + Γ^l_φφ pφ²)
+        dp[1] = l * __builtin_fmaf(pth, pth, (s * s) * pph2);
+        dp[2] = __builtin_fmaf((-2.0f * lr) * pl, pth, (s * c) * pph2);
+        dp[3] = (-2.0f * pph) * __builtin_fmaf(lr, pl, cot * pth);
     }
     GEON_D static void reproject(const FParams& P, float* y, float s) {
-#pragma clang fp contract(fast)
-        const float r2 = P.b0sq + y[1] * y[1];
-        y[4] = fsqrt(y[5] * y[5] + r2 * (y[6] * y[6] + s * s * y[7] * y[7]));
+        const float r2 = __builtin_fmaf(y[1], y[1], P.b0sq);
+        const float ang = __builtin_fmaf(y[6], y[6], (s * s) * (y[7] * y[7]));
+        y[4] = fsqrt(__builtin_fmaf(y[5], y[5], r2 * ang));
     }
     GEON_D static int terminate(const FParams& P, const float* x) {
         const float l = x[1];
@@ -244,66 +249,80 @@ struct AlcubierreF {
     // on the distance covers the FP32 rounding of K ≤ 1024 positions
     // (≤ 1024·2^-24·r).
     GEON_D static uint32_t flat_steps(const FParams& P, const float* y, float h) {
-#pragma clang fp contract(fast)
         const float dx = y[1] - P.ship_x, dy = y[2], dz = y[3];
-        const float r = fsqrt(dx * dx + dy * dy + dz * dz);
+        const float r = fsqrt(norm3sq(dx, dy, dz));
         const float d = r - P.flat_rs;
-        const float p2 = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
-        if (!(d > 0.0f && d * d > h * h * p2)) return 0;   // one step
-        const float k = (d - 1e-3f * (1.0f + r)) * frcp(fabsf(h) * fsqrt(p2) * 1.001f);
+        const float p2 = norm3sq(y[5], y[6], y[7]);
+        if (!(d > 0.0f && d * d > (h * h) * p2)) return 0;   // one step
+        const float k = __builtin_fmaf(-1e-3f, 1.0f + r, d) * frcp((fabsf(h) * fsqrt(p2)) * 1.001f);
         return k >= 1024.0f ? 1024u : (k >= 1.0f ? (uint32_t)k : 1u);
     }
     GEON_D static void shape(const FParams& P, const float* y, float& f, float& fp, float* n) {
-#pragma clang fp contract(fast)
         const float dx = y[1] - P.ship_x, dy = y[2], dz = y[3];
-        const float rs = fsqrt(dx * dx + dy * dy + dz * dz);
+        const float rs = fsqrt(norm3sq(dx, dy, dz));
         const float inv_rs = frcp(fmaxf(rs, 1e-30f));
         n[0] = dx * inv_rs; n[1] = dy * inv_rs; n[2] = dz * inv_rs;
         const float tp = ftanh((rs + P.R) * P.inv_s);
         const float tm = ftanh((rs - P.R) * P.inv_s);
         f = (tp - tm) * P.denom_inv;
-        fp = ((1.0f - tp * tp) - (1.0f - tm * tm)) * P.inv_denom_s;
+        fp = (__builtin_fmaf(-tp, tp, 1.0f) - __builtin_fmaf(-tm, tm, 1.0f)) * P.inv_denom_s;
     }
     GEON_D static void accel(const FParams& P, const float* y, float* dp, float, float) {
-#pragma clang fp contract(fast)
         float f, fp, n[3];
         shape(P, y, f, fp, n);
         const float v = P.v;
         const float p0 = y[4], p1 = y[5], p2 = y[6], p3 = y[7];
-        const float np = n[0] * p1 + n[1] * p2 + n[2] * p3;
-        const float Ac = 2.0f * v * v * f * fp;   // A_i = Ac n_i
-        const float Bc = -v * fp;                 // B_i = Bc n_i
-        const float Q0 = 2.0f * p0 * Ac * np + 2.0f * p1 * Bc * np;
-        const float Q1 = -Ac * n[0] * p0 * p0 + 2.0f * p0 * Bc * (np - n[0] * p1);
+        const float np = __builtin_fmaf(n[0], p1, __builtin_fmaf(n[1], p2, n[2] * p3));
+        const float Ac = (((2.0f * v) * v) * f) * fp;   // A_i = Ac n_i
+        const float Bc = -v * fp;                       // B_i = Bc n_i
+        const float Q0 = __builtin_fmaf((2.0f * p0) * Ac, np, ((2.0f * p1) * Bc) * np);
+        const float Q1 = __builtin_fmaf((-Ac * n[0]) * p0, p0,
+                                        ((2.0f * p0) * Bc) * __builtin_fmaf(-n[0], p1, np));
         const float vf = v * f;
-        dp[0] = 0.5f * (Q0 + vf * Q1);
-        dp[1] = 0.5f * (vf * Q0 - (1.0f - vf * vf) * Q1);
-        dp[2] = 0.5f * (Ac * n[1] * p0 * p0 + 2.0f * Bc * n[1] * p0 * p1);
-        dp[3] = 0.5f * (Ac * n[2] * p0 * p0 + 2.0f * Bc * n[2] * p0 * p1);
+        const float b2 = (2.0f * Bc) * p0 * p1;
+        dp[0] = 0.5f * __builtin_fmaf(vf, Q1, Q0);
+        dp[1] = 0.5f * __builtin_fmaf(vf, Q0, -(__builtin_fmaf(-vf, vf, 1.0f) * Q1));
+        dp[2] = 0.5f * (n[1] * __builtin_fmaf(Ac * p0, p0, b2));
+        dp[3] = 0.5f * (n[2] * __builtin_fmaf(Ac * p0, p0, b2));
     }
     GEON_D static void reproject(const FParams& P, float* y) {
-#pragma clang fp contract(fast)
         float f, fp, n[3];
         shape(P, y, f, fp, n);
         const float vf = P.v * f;
-        const float a = -1.0f + vf * vf;          // g_tt
+        const float a = __builtin_fmaf(vf, vf, -1.0f);   // g_tt
         if (a < -1e-3f) {                         // outside any horizon of the wall
             const float b = -vf * y[5];           // g_tx p^x
-            const float c = y[5] * y[5] + y[6] * y[6] + y[7] * y[7];
-            const float disc = b * b - a * c;
+            const float c = norm3sq(y[5], y[6], y[7]);
+            const float disc = __builtin_fmaf(b, b, -(a * c));
             if (disc >= 0.0f) y[4] = (b + fsqrt(disc)) * frcp(-a);
         }
     }
     GEON_D static int terminate(const FParams& P, const float* x) {
-#pragma clang fp contract(fast)
-        const float r2 = x[1] * x[1] + x[2] * x[2] + x[3] * x[3];
+        const float r2 = norm3sq(x[1], x[2], x[3]);
         if (!__builtin_isfinite(r2)) return DIVERGED;
         return (r2 > P.esc2) ? ESCAPED : RUNNING;
     }
-    GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float, float) {
-#pragma clang fp contract(fast)
-        if (cls == DIVERGED) return true;
-        if (cls == ESCAPED) return y[1] * y[1] + y[2] * y[2] + y[3] * y[3] > 4.0f * P.esc2;
+    // The second tracker counts the steps that evaluated Γ (wall steps).
+    // Rays that stay in the wall region long — those that cross the bubble's
+    // interior, or are caught near its walls — have escape directions that
+    // amplify FP32 rounding (up to radians), and trapped ones change class
+    // between escaped and max-steps. A ray that took more than kWallSteps
+    // such steps, or is still inside the escape sphere at max_steps, is
+    // re-traced in FP64. Measured on the 94 Alcubierre frames of the
+    // 256-frame ragged batch (v ∈ [0.1, 0.9], R = 2, s = 1/8), FP32-mixed
+    // against FP64: escape-angle error max 2.0 rad with 1.5 M of 54 M pixels
+    // above 1e-3 rad at 1024; 0.52 rad and 1.2 M at 256; 1.9e-4 rad and none
+    // at 128 (10.7 % of the rays re-traced); no class mismatch from 128 to
+    // 1024 once max-steps rays are re-traced.
+    static constexpr bool counts_wall = true;
+#ifndef GEON_FP32_WALL_STEPS
+#define GEON_FP32_WALL_STEPS 128
+#endif
+    enum { kWallSteps = GEON_FP32_WALL_STEPS };
+    GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float, float wall) {
+        if (cls == DIVERGED || cls == MAXSTEPS) return true;
+        if (cls == ESCAPED)
+            return norm3sq(y[1], y[2], y[3]) > 4.0f * P.esc2 || wall > (float)kWallSteps;
         return false;
     }
     GEON_D static float radius(const float*) { return 1e30f; }
@@ -313,16 +332,16 @@ struct AlcubierreF {
 // from their Taylor series (|δ| ≤ 0.25: truncation below 2e-8), falling back
 // to sincosf for a large stage increment.
 GEON_D void rotate_sc(float s, float c, float theta_new, float d, float& so, float& co) {
-#pragma clang fp contract(fast)
     if (fabsf(d) > 0.25f) {
         sincosf(theta_new, &so, &co);
         return;
     }
     const float d2 = d * d;
-    const float sd = d * (1.0f - d2 * (1.0f / 6.0f - d2 * (1.0f / 120.0f)));
-    const float cd = 1.0f - d2 * (0.5f - d2 * (1.0f / 24.0f - d2 * (1.0f / 720.0f)));
-    so = s * cd + c * sd;
-    co = c * cd - s * sd;
+    const float sd = d * __builtin_fmaf(-d2, __builtin_fmaf(-d2, 1.0f / 120.0f, 1.0f / 6.0f), 1.0f);
+    const float cd = __builtin_fmaf(
+        -d2, __builtin_fmaf(-d2, __builtin_fmaf(-d2, 1.0f / 720.0f, 1.0f / 24.0f), 0.5f), 1.0f);
+    so = __builtin_fmaf(s, cd, c * sd);
+    co = __builtin_fmaf(c, cd, -(s * sd));
 }
 
 // FP32 RK4 step with the reference tableau. (s, c) carry sin/cos of y[2] in
@@ -346,7 +365,6 @@ struct FlatRunF {
 template <class MF>
 GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, float& s, float& c,
                         FlatRunF& fr) {
-#pragma clang fp contract(fast)
     if constexpr (MF::flat_test != 0) {
         // Γ = 0 at every stage: the RK4 step is the straight line y + h·p
         // (momenta, p^t included, unchanged)
@@ -374,7 +392,7 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
             const float jh = (float)fr.j * h;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                out[i] = fr.a[i] + jh * y[4 + i];
+                out[i] = __builtin_fmaf(jh, y[4 + i], fr.a[i]);
                 out[4 + i] = y[4 + i];
             }
             return 1.0f;
@@ -389,28 +407,28 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
     for (int i = 0; i < 4; ++i) {
         acc[i] = y[4 + i];
         acc[4 + i] = dp[i];
-        yt[i] = y[i] + hh * y[4 + i];
-        yt[4 + i] = y[4 + i] + hh * dp[i];
+        yt[i] = __builtin_fmaf(hh, y[4 + i], y[i]);
+        yt[4 + i] = __builtin_fmaf(hh, dp[i], y[4 + i]);
     }
     if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], hh * y[6], ss, cs);
     MF::accel(P, yt, dp, ss, cs);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float kx = yt[4 + i];
-        acc[i] += 2.0f * kx;
-        acc[4 + i] += 2.0f * dp[i];
-        yt[i] = y[i] + hh * kx;
-        yt[4 + i] = y[4 + i] + hh * dp[i];
+        acc[i] = __builtin_fmaf(2.0f, kx, acc[i]);
+        acc[4 + i] = __builtin_fmaf(2.0f, dp[i], acc[4 + i]);
+        yt[i] = __builtin_fmaf(hh, kx, y[i]);
+        yt[4 + i] = __builtin_fmaf(hh, dp[i], y[4 + i]);
     }
     if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], yt[2] - y[2], ss, cs);
     MF::accel(P, yt, dp, ss, cs);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const float kx = yt[4 + i];
-        acc[i] += 2.0f * kx;
-        acc[4 + i] += 2.0f * dp[i];
-        yt[i] = y[i] + h * kx;
-        yt[4 + i] = y[4 + i] + h * dp[i];
+        acc[i] = __builtin_fmaf(2.0f, kx, acc[i]);
+        acc[4 + i] = __builtin_fmaf(2.0f, dp[i], acc[4 + i]);
+        yt[i] = __builtin_fmaf(h, kx, y[i]);
+        yt[4 + i] = __builtin_fmaf(h, dp[i], y[4 + i]);
     }
     if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], yt[2] - y[2], ss, cs);
     MF::accel(P, yt, dp, ss, cs);
@@ -421,14 +439,14 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
         acc[4 + i] += dp[i];
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
+    for (int i = 0; i < 8; ++i) out[i] = __builtin_fmaf(sixth, acc[i], y[i]);
     if constexpr (MF::uses_theta) {
         sincosf_cw(out[2], &s, &c);
         MF::reproject(P, out, s);
         return fabsf(s);
     } else {
         MF::reproject(P, out);
-        return 1.0f;
+        return 2.0f;   // a step that evaluated Γ (flat steps return 1)
     }
 }
 
@@ -450,7 +468,6 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
 // knows the flatness test's outcome).
 template <class MF>
 GEON_D uint32_t flat_span_f(const FParams& P, float* y, float h, FlatRunF& fr, uint32_t cap) {
-#pragma clang fp contract(fast)
     uint32_t run = 1u << 22;
     if constexpr (MF::flat_test == 2) {
         run = fr.left;
@@ -468,13 +485,13 @@ GEON_D uint32_t flat_span_f(const FParams& P, float* y, float h, FlatRunF& fr, u
 #pragma unroll
         for (int i = 0; i < 4; ++i) fr.a[i] = y[i];
     }
-    const float ra = fsqrt(fr.a[1] * fr.a[1] + fr.a[2] * fr.a[2] + fr.a[3] * fr.a[3]);
-    const float ps = fsqrt(y[5] * y[5] + y[6] * y[6] + y[7] * y[7]);
-    const float avail = fsqrt(P.esc2) * (1.0f - 1e-5f) - ra * (1.0f + 1e-5f);
+    const float ra = fsqrt(norm3sq(fr.a[1], fr.a[2], fr.a[3]));
+    const float ps = fsqrt(norm3sq(y[5], y[6], y[7]));
+    const float avail = __builtin_fmaf(-ra, 1.0f + 1e-5f, fsqrt(P.esc2) * (1.0f - 1e-5f));
     // j stays below 2^22, where (float)j is exact
-    const float jlim = fminf(avail / (fabsf(h) * ps * (1.0f + 1e-5f)), 4194304.0f);
+    const float jlim = fminf(avail / ((fabsf(h) * ps) * (1.0f + 1e-5f)), 4194304.0f);
     if (!(avail > 0.0f && avail < 1e30f) || !(jlim >= (float)fr.j + 1.0f) ||
-        !(fabsf(fr.a[0]) + 8388608.0f * fabsf(h * y[4]) < 1e30f))
+        !(__builtin_fmaf(8388608.0f, fabsf(h * y[4]), fabsf(fr.a[0])) < 1e30f))
         return 0;
     uint32_t m = (uint32_t)jlim - fr.j;
     if (m > run) m = run;
@@ -483,7 +500,7 @@ GEON_D uint32_t flat_span_f(const FParams& P, float* y, float h, FlatRunF& fr, u
     if constexpr (MF::flat_test == 2) fr.left = run - m;
     const float jh = (float)fr.j * h;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) y[i] = fr.a[i] + jh * y[4 + i];
+    for (int i = 0; i < 4; ++i) y[i] = __builtin_fmaf(jh, y[4 + i], fr.a[i]);
     return m;
 }
 

@@ -169,7 +169,16 @@ struct Prec<MP, double> {
     GEON_D static float radius(const double*) { return 1e30f; }
     GEON_D static bool finite(const double* y) { return finite8(y); }
     GEON_D static bool suspicious(const Params&, int, const double*, float, float) { return false; }
+    GEON_D static void track_init(float&, float&) {}
+    GEON_D static void track(float&, float&, float, const double*) {}
 };
+
+// FP32 metrics that count their wall steps in the second tracker
+template <class MF, class = void>
+struct CountsWall : std::false_type {};
+template <class MF>
+struct CountsWall<MF, std::void_t<decltype(MF::counts_wall)>>
+    : std::integral_constant<bool, MF::counts_wall> {};
 
 template <class MP>
 struct Prec<MP, float> {
@@ -218,6 +227,20 @@ struct Prec<MP, float> {
     GEON_D static bool suspicious(const Params& P, int cls, const float* y, float min_s, float min_r) {
         return MF::suspicious(P, cls, y, min_s, min_r);
     }
+    // trackers: min |sin θ| and min radius along the path, or (metrics that
+    // count wall steps) the number of steps that evaluated Γ
+    GEON_D static void track_init(float& min_s, float& min_r) {
+        min_s = 1.0f;
+        min_r = CountsWall<MF>::value ? 0.0f : 1e30f;
+    }
+    GEON_D static void track(float& min_s, float& min_r, float asn, const float* yn) {
+        if constexpr (CountsWall<MF>::value) {
+            if (asn > 1.0f) min_r += 1.0f;   // exact below 2^24 steps
+        } else {
+            min_s = fminf(min_s, asn);
+            min_r = fminf(min_r, MF::radius(yn));
+        }
+    }
 };
 
 template <class MP, class T, int RPL>
@@ -250,8 +273,7 @@ __device__ void sched_run(const geon_sched_args& A) {
         h[r] = (T)J.F.step;
         early[r] = J.early_exit;
         P[r] = PR::params(J.F.m);
-        min_s[r] = 1.0f;
-        min_r[r] = 1e30f;
+        PR::track_init(min_s[r], min_r[r]);
         fr[r] = typename PR::Run{};
         act[r] = true;
         if (!resume) {
@@ -402,10 +424,7 @@ __device__ void sched_run(const geon_sched_args& A) {
                 for (int c = 0; c < 8; ++c) y[r][c] = yn[c];
                 sn[r] = s2;
                 cs[r] = c2;
-                if constexpr (std::is_same<T, float>::value) {
-                    min_s[r] = fminf(min_s[r], asn);
-                    min_r[r] = fminf(min_r[r], PR::radius(yn));
-                }
+                PR::track(min_s[r], min_r[r], asn, yn);
                 ++n[r];
                 ++executed;
             }

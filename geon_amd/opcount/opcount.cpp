@@ -135,14 +135,21 @@ inline cnt_f cnt_rcpf(cnt_f a) { ++T.div; return cnt_f(1.0f / a.v); }
 inline cnt_f cnt_sqrtf(cnt_f a) { ++T.sqrt; return cnt_f(::sqrtf(a.v)); }
 inline cnt_f cnt_expf(cnt_f a) { ++T.tanh; return cnt_f(::expf(a.v)); }
 inline bool cnt_isfinite(cnt_f a) { return ::isfinite(a.v); }
+inline cnt_f cnt_fmaf(cnt_f a, cnt_f b, cnt_f c) {
+    ++T.mul;
+    ++T.add;
+    return cnt_f(::fmaf(a.v, b.v, c.v));
+}
 
 #define __builtin_amdgcn_rcpf(x) cnt_rcpf(x)
+#define __builtin_fmaf(a, b, c) cnt_fmaf(a, b, c)
 #define __builtin_sqrtf(x) cnt_sqrtf(x)
 #define __expf(x) cnt_expf(x)
 #define __builtin_isfinite(x) cnt_isfinite(x)
 #define float cnt_f
 #include "../csrc/kernels/geon_fp32.hpp"
 #undef float
+#undef __builtin_fmaf
 #undef __builtin_amdgcn_rcpf
 #undef __builtin_sqrtf
 #undef __expf

@@ -120,6 +120,29 @@ void or_metric_default(int32_t kind, geon_metric_t* out) {
     }
 }
 
+uint64_t or_sky_hash1(uint64_t x);
+
+/* A planted perturbation for conditioning tests only (0 = the reference):
+ * with k = ±1 every tanh of the shape function below is moved by one ulp
+ * towards ±∞; with |k| ≥ 2 each evaluation is moved by one ulp up or down,
+ * the direction drawn from a hash of its argument and k. A test measures how
+ * far the reference's own results move when its tanh changes in the last
+ * bit. */
+static int or_tanh_ulp_shift = 0;
+void or_set_tanh_ulp_shift(int k) { or_tanh_ulp_shift = k; }
+static double shape_tanh(double v) {
+    const double t = tanh(v);
+    const int k = or_tanh_ulp_shift;
+    if (k == 0) return t;
+    int up = k > 0;
+    if (k >= 2 || k <= -2) {
+        uint64_t bits;
+        memcpy(&bits, &v, sizeof bits);
+        up = (int)(or_sky_hash1(bits ^ ((uint64_t)(int64_t)k * 0x9E3779B97F4A7C15ull)) & 1u);
+    }
+    return nextafter(t, up ? INFINITY : -INFINITY);
+}
+
 /* alcubierre.rs:33-44 — smooth top-hat f(r_s). */
 static double alc_shape(const geon_metric_t* m, const double x[4]) {
     double dx = x[1] - m->p[3];
@@ -128,8 +151,8 @@ static double alc_shape(const geon_metric_t* m, const double x[4]) {
     double r_s = sqrt(dx * dx + dy * dy + dz * dz);
     double s = m->p[1];
     double r = m->p[0];
-    double t_plus = tanh((r_s + r) / s);
-    double t_minus = tanh((r_s - r) / s);
+    double t_plus = shape_tanh((r_s + r) / s);
+    double t_minus = shape_tanh((r_s - r) / s);
     double denom = 2.0 * tanh(r / s);
     return (t_plus - t_minus) / denom;
 }

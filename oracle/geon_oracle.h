@@ -33,6 +33,11 @@ double or_dot4(const double g[16], const double a[4], const double b[4]);
 int    or_inv4(const double m[16], double out[16]);
 
 /* L1 metric */
+/* Conditioning tests only: shift every tanh of the Alcubierre shape function
+ * by one ulp towards ±∞ (k = ±1), or each evaluation up or down by one ulp
+ * as a hash of its argument and k decides (|k| ≥ 2); 0 restores the
+ * reference. Not thread-safe against traces in flight. */
+void   or_set_tanh_ulp_shift(int k);
 void   or_metric_default(int32_t kind, geon_metric_t* out);
 void   or_g(const geon_metric_t* m, const double x[4], double g[16]);
 void   or_dg(const geon_metric_t* m, const double x[4], double dg[64]);

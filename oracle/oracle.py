@@ -338,6 +338,14 @@ def curvature(m, x_aos, what, nthreads=THREADS):
     return out, el
 
 
+def set_tanh_ulp_shift(k: int) -> None:
+    """Conditioning tests only: move every tanh of the Alcubierre shape function
+    by one ulp towards ±∞ (k = ±1), or each evaluation up or down by one ulp as
+    a hash of its argument and k decides (|k| ≥ 2); 0 restores the reference's
+    arithmetic."""
+    lib().or_set_tanh_ulp_shift(C.c_int(int(k)))
+
+
 def equatorial_ray(mass, r_obs, alpha):
     y = np.zeros(8)
     lib().or_equatorial_ray(mass, r_obs, alpha, _dp(y))

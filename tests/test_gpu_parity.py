@@ -1,25 +1,20 @@
 """GPU parity: the HIP path through the C ABI vs the CPU oracle on the same inputs.
 
-Bar (FP64): per-ray outcome class identical; escape directions within 1e-9 rad;
-step counts identical. Two documented exceptions, both properties of the
-reference algorithm itself rather than of the port:
-  * rays within 1e-9 of the capture boundary in impact parameter;
-  * fixed-step blow-ups at the Schwarzschild horizon: an RK4 stage landing on
-    r ≈ 2M makes f → 0 and flings the state to a huge finite r that the next
-    terminate() reads as Escaped. The final state of such a ray is rounding
-    noise amplified by ~1/f², so only its class is compared.
-The device transcendentals (ocml sin/cos/tanh) may differ from glibc by an
-ulp; everything else in the FP64 kernels follows the reference's operation
-order without FMA contraction.
+The FP64 bar is parity_util's (stated there): classes identical on every ray
+except rays shown to lie within 1e-9 of a capture boundary in impact
+parameter, step counts identical, escape directions within 1e-9 rad, null
+residuals equal to the oracle's (and ≤ 1e-12 wherever the oracle's is). The
+device code follows the reference's operation order without FMA
+contraction; its sin/cos/tanh may differ from glibc's in the last bit.
 """
 import math
 
 import numpy as np
 import pytest
 
-pytestmark = pytest.mark.gpu
+from parity_util import ANGLE_TOL, angle as _angle, compare_rays
 
-ANGLE_TOL = 1e-9
+pytestmark = pytest.mark.gpu
 
 
 @pytest.fixture(scope="module")
@@ -30,29 +25,9 @@ def ctx():
     c.close()
 
 
-def _angle(a, b):
-    cross = np.linalg.norm(np.cross(a, b), axis=-1)
-    dot = (a * b).sum(axis=-1)
-    return np.arctan2(cross, dot)
-
-
-def _compare_rays(O, om, y0, got, ref, esc_r, label):
-    """Return (class mismatches, max angle) after the documented exclusions."""
-    artefact = (ref["cls"] == O.ESCAPED) & (np.abs(ref["y"][:, 1]) > 10 * esc_r)
-    same = got["cls"] == ref["cls"]
-    bad = ~same & ~artefact
-    ok = same & ~artefact
-    assert np.array_equal(got["steps"][ok].astype(np.int64), ref["steps"][ok].astype(np.int64)), label
-    esc = ok & (ref["cls"] == O.ESCAPED)
-    ang = _angle(got["dir"][esc], ref["dir"][esc]) if esc.any() else np.zeros(1)
-    print(f"{label}: rays {len(same)}, class mismatches {int((~same).sum())} "
-          f"(artefacts {int(artefact.sum())}), max angle {ang.max():.3e}", flush=True)
-    return int(bad.sum()), float(ang.max())
-
-
 def test_render_schwarzschild_c1(ctx, O):
     """Plain FP64 Schwarzschild render, 256², camera r=30M, 60°, capture r<2.01M,
-    escape r>100M, step 0.05, max 4000, stars on."""
+    escape r>100M, step 0.05, max 4000, stars on: every one of the 65,536 rays."""
     import geon_amd as geon
     metric = geon.Schwarzschild(mass=1.0, horizon_pad=1.005, escape_radius=100.0)
     cam = geon.Camera((0.0, 30.0, math.pi / 2, 0.0), 60.0, 256, 256)
@@ -62,18 +37,18 @@ def test_render_schwarzschild_c1(ctx, O):
     ocam = O.camera(cam.observer, 60.0, 256, 256)
     y0 = O.camera_rays(om, ocam)
     ref = O.trace_rays(om, y0, 0.05, 4000)
-    got = dict(cls=cls, dir=d)
-    # per-ray steps/states come from trace_rays on the same initial states
+    # per-ray steps, states and residuals from trace_rays on the same initial states
     tr = geon.trace_rays(metric, y0, 0.05, 4000, ctx=ctx)
     assert np.array_equal(tr["cls"], cls)
-    got["steps"] = tr["steps"]
-    bad, ang = _compare_rays(O, om, y0, got, ref, 100.0, "C1 render")
-    assert bad == 0
-    assert ang <= ANGLE_TOL
+    assert np.array_equal(tr["dir"], d)
+    compare_rays(O, om, y0, tr, ref, 0.05, 4000, "C1 render", esc_r=100.0)
     # statistics follow render.rs:146-163
     assert stats["captured"] + stats["escaped"] + stats["diverged"] == 65536
     assert stats["ray_steps"] == int(tr["steps"].astype(np.int64).sum())
+    assert stats["gamma_steps"] == stats["ray_steps"]       # Γ at every Schwarzschild step
     assert stats["avg_steps"] == stats["ray_steps"] / 65536
+    fin = np.isfinite(tr["resid"])
+    assert stats["max_null_residual"] == tr["resid"][fin].max()
     # colour of escaped pixels equals the sky at the reference direction, up to
     # ulp-level differences in acos/atan2 at cell edges
     esc = (cls == O.ESCAPED) & (ref["cls"] == O.ESCAPED)
@@ -82,38 +57,168 @@ def test_render_schwarzschild_c1(ctx, O):
     assert np.all(rgb[cls == O.CAPTURED] == 0.0)
 
 
-@pytest.mark.parametrize("kind,kw,obs,esc_r,max_steps,chaotic", [
-    (3, dict(throat_radius=1.0), (0.0, 15.0, math.pi / 2, 0.0), 60.0, 3000, False),
-    (2, dict(bubble_radius=2.0, bubble_thickness=0.125, ship_velocity=0.5), (0.0, 10.0, 0.0, 0.0),
-     60.0, 3000, False),
-    (2, dict(bubble_radius=2.0, bubble_thickness=0.5, ship_velocity=2.0), (0.0, 8.0, 0.0, 0.0),
-     60.0, 3000, True),
-    (0, {}, (0.0, 20.0, 0.0, 0.0), 60.0, 3000, False),
-    (1, dict(mass=1.0), (0.0, 20.0, 1.3, 0.4), 80.0, 6000, False),
+@pytest.mark.parametrize("kind,kw,obs", [
+    (0, {}, (0.0, 20.0, 0.0, 0.0)),
+    (1, dict(mass=1.0), (0.0, 30.0, math.pi / 2, 0.0)),
+    (1, dict(mass=1.0), (0.0, 12.0, 1.3, 0.4)),
+    (2, dict(bubble_thickness=0.125, ship_velocity=0.5), (0.0, -10.0, 0.0, 0.0)),
+    (2, {}, (0.0, 8.0, 0.0, 0.0)),
+    (3, dict(throat_radius=1.0), (0.0, 15.0, math.pi / 2, 0.0)),
 ])
-def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps, chaotic):
-    """chaotic: the superluminal bubble (v = 2) has horizons in its walls, where
-    the reference's fixed step amplifies ulp-level differences of tanh between
-    libm implementations; there the bound is statistical (90% of escaped rays
-    within 1e-9 rad, all within 1e-6 rad) and classes must still agree."""
+def test_camera_rays_are_null(ctx, O, kind, kw, obs):
+    """Camera rays leave the observer on the null cone: |g_μν p^μ p^ν| ≤ 1e-12 at
+    the initial state (trace with max_steps = 0 reports the residual of the
+    state it was given), equal to the oracle's residual of the same states."""
+    import geon_amd as geon
+    om = O.metric(kind, **kw)
+    y0 = O.camera_rays(om, O.camera(obs, 60.0, 48, 40))
+    got = geon.trace_rays(om, y0, 0.05, 0, ctx=ctx)
+    ref = O.trace_rays(om, y0, 0.05, 0)
+    assert np.array_equal(got["y"], y0)
+    assert got["resid"].max() <= 1e-12 and ref["resid"].max() <= 1e-12
+    assert np.abs(got["resid"] - ref["resid"]).max() <= 1e-15
+
+
+@pytest.mark.parametrize("kind,kw,obs,esc_r,max_steps", [
+    (3, dict(throat_radius=1.0), (0.0, 15.0, math.pi / 2, 0.0), 60.0, 3000),
+    (2, dict(bubble_radius=2.0, bubble_thickness=0.125, ship_velocity=0.5), (0.0, 10.0, 0.0, 0.0),
+     60.0, 3000),
+    (0, {}, (0.0, 20.0, 0.0, 0.0), 60.0, 3000),
+    (1, dict(mass=1.0), (0.0, 20.0, 1.3, 0.4), 80.0, 6000),
+    (1, dict(mass=1.0), (0.0, 12.0, 1.3, 0.4), 80.0, 8000),
+])
+def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps):
     import geon_amd as geon
     om = O.metric(kind, escape_radius=esc_r, **kw)
-    cam = O.camera(obs, 60.0, 64, 48)
-    y0 = O.camera_rays(om, cam)
+    y0 = O.camera_rays(om, O.camera(obs, 60.0, 64, 48))
     ref = O.trace_rays(om, y0, 0.05, max_steps)
     got = geon.trace_rays(om, y0, 0.05, max_steps, ctx=ctx)
-    bad, ang = _compare_rays(O, om, y0, got, ref, esc_r, f"trace kind={kind} {kw}")
-    assert bad == 0
-    if not chaotic:
-        assert ang <= ANGLE_TOL
-        fin = np.isfinite(ref["y"]).all(axis=1) & (got["cls"] == ref["cls"])
-        rel = np.abs(got["y"][fin] - ref["y"][fin]) / np.maximum(1.0, np.abs(ref["y"][fin]))
-        assert rel.max() < 1e-8
-    else:
-        esc = (got["cls"] == ref["cls"]) & (ref["cls"] == O.ESCAPED) & \
-            (np.abs(ref["y"][:, 1:4]).max(axis=1) < 10 * esc_r)
-        a = _angle(got["dir"][esc], ref["dir"][esc])
-        assert (a <= ANGLE_TOL).mean() >= 0.90 and a.max() <= 1e-6
+    compare_rays(O, om, y0, got, ref, 0.05, max_steps, f"trace kind={kind} {kw} obs={obs}",
+                 esc_r=esc_r)
+    # final states of escaped rays (rays still orbiting at max_steps, or
+    # captured next to the horizon, amplify last-bit differences without bound)
+    far = np.abs(ref["y"][:, 1]) if kind in (1, 3) else np.linalg.norm(ref["y"][:, 1:4], axis=1)
+    fin = (got["cls"] == ref["cls"]) & (ref["cls"] == O.ESCAPED) & (far < 10 * esc_r)
+    rel = np.abs(got["y"][fin] - ref["y"][fin]) / np.maximum(1.0, np.abs(ref["y"][fin]))
+    assert rel.max() < 1e-8
+
+
+def test_superluminal_alcubierre_conditioning(ctx, O):
+    """The superluminal bubble (v = 2, s = 0.5) has horizons in its walls: there
+    the reference's own result is ill-conditioned in the last bit of tanh.
+    Classes and step counts must agree on every ray. Escape directions agree
+    within 1e-9 rad except on rays that are ill-conditioned in that last bit,
+    shown per ray with perturbations planted in the oracle: every tanh moved
+    by one ulp up, down, or up or down at random per evaluation (the device
+    tanh differs from glibc's by about that). Such a ray's own one-ulp
+    spread is above 1e-10 rad and the GPU lies within 30× of it."""
+    import geon_amd as geon
+    om = O.metric(O.ALCUBIERRE, bubble_radius=2.0, bubble_thickness=0.5, ship_velocity=2.0,
+                  escape_radius=60.0)
+    y0 = O.camera_rays(om, O.camera((0.0, 8.0, 0.0, 0.0), 60.0, 64, 48))
+    ref = O.trace_rays(om, y0, 0.05, 3000)
+    got = geon.trace_rays(om, y0, 0.05, 3000, ctx=ctx)
+    assert np.array_equal(got["cls"], ref["cls"])
+    assert np.array_equal(got["steps"].astype(np.int64), ref["steps"].astype(np.int64))
+    esc = (ref["cls"] == O.ESCAPED) & (np.linalg.norm(ref["y"][:, 1:4], axis=1) < 600.0)
+    a = _angle(got["dir"][esc], ref["dir"][esc])
+    try:
+        sens = np.zeros(esc.sum())
+        for k in (-1, 1, 2, 3, 4, 5):
+            O.set_tanh_ulp_shift(k)
+            pert = O.trace_rays(om, y0[esc], 0.05, 3000)
+            sens = np.maximum(sens, _angle(pert["dir"], ref["dir"][esc]))
+    finally:
+        O.set_tanh_ulp_shift(0)
+    off = a > ANGLE_TOL
+    ratio = a[off] / np.maximum(sens[off], 1e-300)
+    print(f"superluminal: escaped {esc.sum()}, above 1e-9 rad {off.sum()} (max {a.max():.3e}); "
+          f"their one-ulp spread min {sens[off].min() if off.any() else 0:.3e}, GPU/spread max "
+          f"{ratio.max() if off.any() else 0:.2f}; rays whose spread exceeds 1e-9 rad "
+          f"{(sens > ANGLE_TOL).sum()}", flush=True)
+    assert (sens[off] > 1e-10).all() and (ratio <= 30.0).all()
+
+
+def _c2_sample(O, om, cam, n, seed):
+    st, tet = O.tetrad(om, cam.observer)
+    assert st == 0
+    idx = np.random.default_rng(seed).choice(cam.width * cam.height, size=n, replace=False)
+    y0 = np.array([O.ray_state_sub(cam, tet, int(i) % cam.width, int(i) // cam.width) for i in idx])
+    return idx, y0
+
+
+@pytest.mark.parametrize("label,kind,kw,obs,max_steps", [
+    ("alcubierre x=-10 (as given)", 2, dict(bubble_radius=2.0, bubble_thickness=0.125,
+                                             ship_velocity=0.5), (0.0, -10.0, 0.0, 0.0), 20_000),
+    ("alcubierre x=+10 (bubble in view)", 2, dict(bubble_radius=2.0, bubble_thickness=0.125,
+                                                   ship_velocity=0.5), (0.0, 10.0, 0.0, 0.0), 40_000),
+    ("morris-thorne l=15", 3, dict(throat_radius=1.0), (0.0, 15.0, math.pi / 2, 0.0), 20_000),
+])
+def test_c2_render_1024_sampled(ctx, O, label, kind, kw, obs, max_steps):
+    """The generic-metric FP64 workload at its configured size: 1024² at 60°,
+    step 0.05, reference escape radius (1000), fd_step 1e-5. The whole frame is
+    rendered on the GPU; 2048 seeded pixels are traced by the oracle and held
+    to the FP64 bar (class, steps, direction, residual), and the frame's
+    statistics are checked against the per-ray outputs."""
+    import geon_amd as geon
+    om = O.metric(kind, **kw)
+    cam = O.camera(obs, 60.0, 1024, 1024)
+    gm = geon.Alcubierre(**kw) if kind == 2 else geon.MorrisThorne(**kw)
+    gcam = geon.Camera(obs, 60.0, 1024, 1024)
+    rgb, stats, cls, d = geon.render(gm, gcam, geon.RenderConfig(step=0.05, max_steps=max_steps),
+                                     ctx=ctx)
+    idx, y0 = _c2_sample(O, om, cam, 2048, seed=kind * 10 + int(obs[1] > 0))
+    print(f"{label}: frame classes {np.bincount(cls, minlength=4).tolist()}, ray-steps "
+          f"{stats['ray_steps']} (Γ-evaluating {stats['gamma_steps']}), kernel "
+          f"{stats['kernel_ms']:.1f} ms", flush=True)
+    ref = O.trace_rays(om, y0, 0.05, max_steps)
+    tr = geon.trace_rays(gm, y0, 0.05, max_steps, ctx=ctx)
+    # the render's pixels are the traced rays (one sample per pixel)
+    assert np.array_equal(cls[idx], tr["cls"])
+    assert np.array_equal(d[idx], tr["dir"])
+    compare_rays(O, om, y0, tr, ref, 0.05, max_steps, label, esc_r=1000.0)
+    assert stats["captured"] + stats["escaped"] + stats["diverged"] == 1024 * 1024
+    assert 0 < stats["gamma_steps"] <= stats["ray_steps"] or kind == 2
+    if kind == 3:
+        assert stats["gamma_steps"] == stats["ray_steps"]
+
+
+def test_c2_curvature_full_launches(ctx, O):
+    """C2 curvature as configured: Alcubierre (R=2, s=1/8, v=0.5) T_μν on the
+    whole 128³ grid over [−4, 4]³ (x_i = −4 + 8i/127, t = 0) in one launch,
+    4096 seeded grid points against the oracle; Morris–Thorne (b0 = 1) on the
+    4096-point line l ∈ [−4, 4], θ = π/2, every point against the oracle.
+    Tolerance 1e-5 of the largest |T| (the FD of an FD amplifies rounding by
+    ~1/h²), and the Morris–Thorne closed form T_tt = −b0²/(8π(b0² + l²)²)."""
+    import geon_amd as geon
+    alc = O.metric(O.ALCUBIERRE, bubble_radius=2.0, bubble_thickness=0.125, ship_velocity=0.5)
+    g = -4.0 + 8.0 * np.arange(128) / 127.0
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([np.zeros(X.size), X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+    T = geon.stress_energy(alc, pts, ctx=ctx)
+    assert T.shape == (128 ** 3, 4, 4) and np.isfinite(T).all()
+    idx = np.random.default_rng(5).choice(len(pts), 4096, replace=False)
+    ref, _ = O.curvature(alc, pts[idx], O.CURV_STRESS)
+    scale = max(1.0, np.abs(ref).max())
+    err = np.abs(T[idx].reshape(-1, 16) - ref).max()
+    print(f"alcubierre 128^3: max |T| {np.abs(T).max():.4e}, sampled max |Δ| {err:.3e} "
+          f"(scale {scale:.3e})", flush=True)
+    assert err <= 1e-5 * scale
+    # T is symmetric; far outside the walls the metric is flat and T vanishes
+    assert np.abs(T - T.transpose(0, 2, 1)).max() <= 1e-5 * scale
+    far = np.linalg.norm(pts[:, 1:], axis=1) > 2.0 + 22 * 0.125
+    assert np.abs(T[far]).max() == 0.0
+
+    mt = O.metric(O.MORRIS_THORNE, throat_radius=1.0)
+    l = -4.0 + 8.0 * np.arange(4096) / 4095.0
+    line = np.stack([np.zeros(4096), l, np.full(4096, math.pi / 2), np.zeros(4096)], axis=1)
+    Tm = geon.stress_energy(mt, line, ctx=ctx)
+    refm, _ = O.curvature(mt, line, O.CURV_STRESS)
+    errm = np.abs(Tm.reshape(-1, 16) - refm).max()
+    print(f"morris-thorne 4096-point line: max |Δ| {errm:.3e}", flush=True)
+    assert errm <= 1e-5 * max(1.0, np.abs(refm).max())
+    closed = -1.0 / (8 * math.pi * (1.0 + l * l) ** 2)
+    assert np.abs(Tm[:, 0, 0] - closed).max() <= 1e-6
 
 
 def test_alcubierre_long_flat_runs(ctx, O):
@@ -126,8 +231,7 @@ def test_alcubierre_long_flat_runs(ctx, O):
     y0 = O.camera_rays(om, O.camera((0.0, 40.0, 0.0, 0.0), 8.0, 48, 48))
     ref = O.trace_rays(om, y0, 0.05, 4000)
     got = geon.trace_rays(om, y0, 0.05, 4000, ctx=ctx)
-    bad, ang = _compare_rays(O, om, y0, got, ref, 100.0, "alcubierre far camera")
-    assert bad == 0 and ang <= ANGLE_TOL
+    compare_rays(O, om, y0, got, ref, 0.05, 4000, "alcubierre far camera", esc_r=100.0)
     # the wall deflected most rays (their final direction is not the initial one)
     p0 = y0[:, 5:8] / np.linalg.norm(y0[:, 5:8], axis=1)[:, None]
     defl = np.arccos(np.clip((p0 * ref["dir"]).sum(axis=1), -1.0, 1.0))

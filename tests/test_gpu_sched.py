@@ -97,6 +97,38 @@ def test_results_independent_of_launch_shape_and_mix(ctx, tune):
             assert np.array_equal(x, y)
 
 
+def test_mixed_precision_independent_of_slot_and_shape(ctx):
+    """FP32-mixed rays give the same bits whichever kernel instantiation (one
+    or two rays per lane) and slot they run in: the FP32 step writes every
+    fused multiply-add out, so its rounding does not depend on how the
+    compiler treats each copy. A mix with a frame in front shifts every
+    ray's queue position and so its slot."""
+    import geon_amd as geon
+    frames = [
+        (geon.MorrisThorne(throat_radius=1.9, escape_radius=100.0),
+         geon.Camera((0.0, 35.0, math.pi / 2, 0.0), 60.0, 160, 120),
+         geon.RenderConfig(step=0.05, max_steps=5000, precision=geon.PREC_MIXED)),
+        (geon.Schwarzschild(mass=1.0, horizon_pad=1.005, escape_radius=100.0),
+         geon.Camera((0.0, 30.0, math.pi / 2, 0.0), 60.0, 128, 96),
+         geon.RenderConfig(step=0.05, max_steps=4000, precision=geon.PREC_MIXED)),
+        (geon.Alcubierre(bubble_radius=2.0, bubble_thickness=0.125, ship_velocity=0.8,
+                         escape_radius=80.0),
+         geon.Camera((0.0, 10.0, 0.0, 0.0), 60.0, 96, 96),
+         geon.RenderConfig(step=0.05, max_steps=5000, precision=geon.PREC_MIXED)),
+    ]
+    lead = (geon.Schwarzschild(mass=1.3, horizon_pad=1.005, escape_radius=90.0),
+            geon.Camera((0.0, 25.0, 1.2, 0.0), 50.0, 37, 29),
+            geon.RenderConfig(step=0.05, max_steps=3000, precision=geon.PREC_MIXED))
+    base, _, _ = _run_frames(ctx, frames, dict(rays_per_lane=2))
+    for tune, mix in ((dict(rays_per_lane=1), False), (dict(rays_per_lane=2), True),
+                      (dict(rays_per_lane=1, refill=5, block=128), True)):
+        res, _, _ = _run_frames(ctx, ([lead] if mix else []) + frames, tune)
+        res = res[1:] if mix else res
+        for a, b in zip(base, res):
+            for x, y in zip(a, b):
+                assert np.array_equal(x, y), (tune, mix)
+
+
 def _angle(a, b):
     cross = np.linalg.norm(np.cross(a, b), axis=-1)
     return np.arctan2(cross, (a * b).sum(axis=-1))
