@@ -1,13 +1,16 @@
 """Benchmark of the geodesic path on one MI355X.
 
-Headline (`value`): RK4 ray-steps/s of the FP64 generic-metric renders —
-Alcubierre R=2, σ=8 (thickness 1/8), v_s=0.5, 1024² from x=-10 (as given:
-the reference camera looks toward −x, away from the bubble) and from x=+10
-(bubble in view, max 40,000 steps); Morris–Thorne b0=1, l=15, 1024², max
-20,000 (reference defaults: nearly every ray runs out of steps). One bench
-step renders all three; rays are generated on the device from the camera and
-outputs stay in HBM. Timing: HIP events recorded on the stream every kernel
-is launched on.
+Headline (`value`): RK4 ray-steps/s of the FP64 generic-metric renders,
+counting the steps that evaluate the Christoffel symbols at their four stages
+(`gamma_steps`) — Alcubierre R=2, σ=8 (thickness 1/8), v_s=0.5, 1024² from
+x=-10 (as given: the reference camera looks toward −x, away from the bubble)
+and from x=+10 (bubble in view, max 40,000 steps); Morris–Thorne b0=1, l=15,
+1024², max 20,000 (reference defaults: nearly every ray runs out of steps).
+Steps through regions where Γ is exactly zero (beyond the Alcubierre wall),
+which the kernels take as spans of additions, are reported separately
+(`all_ray_steps_per_s`). One bench step renders all three; rays are generated
+on the device from the camera and outputs stay in HBM. Timing: HIP events
+recorded on the stream every kernel is launched on.
 
 Also reported on the same run:
   precision_split  Schwarzschild 4096², r=30M, 60°, capture r<2.01M, escape
@@ -16,16 +19,21 @@ Also reported on the same run:
                    mismatch and escape-angle error vs the FP64 frame, plus a
                    CPU-oracle check on a pixel sample
   ragged_batch     256 frames, sides log-uniform 64..2048, random metrics, one
-                   persistent launch per kind, FP64 and FP32-mixed
+                   persistent launch per kind, FP64 and FP32-mixed, with the
+                   mixed frames' class mismatch and escape-angle error against
+                   the FP64 frames
   serving          Poisson arrivals, 80% renders 32²..2048² / 20% bisections
-                   64×40, p50/p99 latency and sustained ray-steps/s of the
-                   FP32-mixed server, next to the plain FP64 kernel on the same
-                   renders
+                   64×40, p50/p99 latency of the FP32-mixed server; at each
+                   rate the plain FP64 kernel runs the same renders one after
+                   another, and a seeded sample of served renders is compared
+                   with those FP64 frames
   curvature        Alcubierre T_μν on 128³ over [-4,4]³, Morris–Thorne on 4096 points
   adaptive         Cash–Karp RK4(5) traces of the Morris–Thorne camera rays at
                    512², next to the fixed-step trace of the same rays (both on
                    device-resident states)
   cpu_baseline     the oracle (reference algorithm in C, -O3, no FMA) on a sample
+                   of the Morris–Thorne frame's pixels, checked against the
+                   GPU frame in HBM
 """
 from __future__ import annotations
 
@@ -41,6 +49,11 @@ sys.path.insert(0, ROOT)
 
 FP64_PEAK_TFLOPS = 78.6
 FP32_PEAK_TFLOPS = 157.3
+METRIC = "RK4 ray-steps/s on 1 GPU; p99 frame latency; FP32-mixed vs FP64 at \u22641e-3 mismatch"
+# Minimal FP64 operations per Morris-Thorne RK4 ray-step (51 add/mul + 5 div +
+# 3 sin per right-hand side, four per step, plus the RK4 combination); the
+# kernels' own formulation, which keeps the reference's rounding, executes 442.
+MT_OPS_MINIMAL = 340
 
 
 def measured_traffic():
@@ -134,7 +147,8 @@ def main():
     stats = None
     for _ in range(max(1, args.warmup)):
         stats = run_once(True)
-    ray_steps_per_step = sum(int(s["ray_steps"]) for s in stats.values())
+    gamma_per_step = sum(int(s["gamma_steps"]) for s in stats.values())
+    all_per_step = sum(int(s["ray_steps"]) for s in stats.values())
     torch.cuda.synchronize()
     t0, t1 = ev(), ev()
     wall0 = time.perf_counter()
@@ -146,21 +160,25 @@ def main():
     wall = time.perf_counter() - wall0
     ms_total = t0.elapsed_time(t1)
     ms_per_step = ms_total / args.steps
-    value = ray_steps_per_step * args.steps / (ms_total * 1e-3)
+    value = gamma_per_step * args.steps / (ms_total * 1e-3)
     final = run_once(True)
     for k in stats:
         assert final[k]["ray_steps"] == stats[k]["ray_steps"], k
+        assert final[k]["gamma_steps"] == stats[k]["gamma_steps"], k
     mt_ms = final["morris_thorne_l15"]["kernel_ms"]
     mt_steps = int(final["morris_thorne_l15"]["ray_steps"])
-    mt_ops = op_counts()["ray_step"]["morris_thorne"]["ops"]
-    achieved = mt_steps * mt_ops / (mt_ms * 1e-3) / 1e12
+    mt_counted = op_counts()["ray_step"]["morris_thorne"]["ops"]
+    achieved = mt_steps * MT_OPS_MINIMAL / (mt_ms * 1e-3) / 1e12
+    achieved_counted = mt_steps * mt_counted / (mt_ms * 1e-3) / 1e12
     roofline = {"bound": "valu_fp64", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4),
                 "traffic": measured_traffic(), "algorithmic_bytes": 40 * 1024 * 1024,
-                "kernel": "geon_render_rays_mt", "ops_per_ray_step": mt_ops,
+                "kernel": "geon_render_rays_mt", "ops_per_ray_step": MT_OPS_MINIMAL,
+                "ops_per_ray_step_executed_formulation": mt_counted,
+                "frac_executed_formulation": round(achieved_counted / FP64_PEAK_TFLOPS, 4),
                 "ray_steps_per_launch": mt_steps, "kernel_ms": round(mt_ms, 3)}
     result = {
-        "metric": "RK4 ray-steps/s on 1 GPU",
+        "metric": METRIC,
         "value": value,
         "unit": "ray-steps/s",
         "n_gpus": 1,
@@ -174,14 +192,20 @@ def main():
         "data": "synthetic (camera rays generated on device)",
         "config": {"workload": "generic-metric FP64 renders: Alcubierre 1024² (x=-10 as given, "
                                "x=+10 bubble in view), Morris-Thorne 1024² (l=15); step 0.05",
-                   "ray_steps_per_step": ray_steps_per_step,
+                   "value_counts": "RK4 ray-steps that evaluate Γ at their four stages",
+                   "ray_steps_per_step": gamma_per_step,
+                   "per_render_gamma_ray_steps": {k: int(v["gamma_steps"]) for k, v in stats.items()},
                    "per_render_ray_steps": {k: int(v["ray_steps"]) for k, v in stats.items()},
                    "per_render_kernel_ms": {k: round(v["kernel_ms"], 3) for k, v in final.items()},
                    "parallelism": "single GPU"},
+        # every ray-step, including those through exactly flat regions (Γ = 0)
+        # that the kernels take as spans of additions
+        "all_ray_steps_per_s": all_per_step * args.steps / (ms_total * 1e-3),
         "roofline": roofline,
         "wall_s": round(wall, 3),
     }
-    log(f"headline: {value:.4e} ray-steps/s, {ms_per_step:.1f} ms per step")
+    log(f"headline: {value:.4e} Γ-evaluating ray-steps/s ({result['all_ray_steps_per_s']:.4e} "
+        f"with flat-span steps), {ms_per_step:.1f} ms per step")
     sections = set() if args.quick else set(args.sections.split(","))
     if "split" in sections:
         result["precision_split"] = precision_split(geon, ctx, torch, stream, outs, ev)
@@ -199,8 +223,18 @@ def main():
         result["adaptive"] = adaptive(geon, ctx, torch, stream, ev, jobs)
         log(f"adaptive: {json.dumps(result['adaptive'])}")
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(jobs)
+        result["cpu_baseline"] = cpu_baseline(jobs, bufs["morris_thorne_l15"])
         log(f"cpu baseline: {json.dumps(result['cpu_baseline'])}")
+    if "serving" in result:
+        r100 = result["serving"]["rates"].get("100") or next(iter(result["serving"]["rates"].values()))
+        result["p99_frame_latency_ms"] = {"rate_req_per_s": r100["rate"],
+                                          "render_p99": r100["render_latency_ms"]["p99"],
+                                          "render_p50": r100["render_latency_ms"]["p50"]}
+    if "precision_split" in result:
+        ps = result["precision_split"]
+        result["fp32_mixed_vs_fp64"] = {"speedup": ps["speedup_vs_plain_fp64"],
+                                        "pixel_class_mismatch": ps["pixel_class_mismatch_vs_fp64"],
+                                        "workload": "Schwarzschild 4096² precision split"}
     print(json.dumps(result))
 
 
@@ -292,16 +326,53 @@ def precision_split(geon, ctx, torch, stream, outs, ev):
     }
 
 
+def _mixed_vs_fp64(torch, cls64, dir64, clsm, dirm, acc):
+    """Accumulate class mismatches and escape-angle errors of one FP32-mixed
+    frame against its FP64 frame (device tensors: classes n, directions SoA 3n)."""
+    acc["pixels"] += int(cls64.numel())
+    acc["class_mismatch"] += int((cls64 != clsm).sum())
+    d64, dm = dir64.view(3, -1).T, dirm.view(3, -1).T
+    both = (cls64 == 2) & (clsm == 2) & (d64.abs().amax(dim=1) > 0) & (dm.abs().amax(dim=1) > 0)
+    if int(both.sum()):
+        a, b = d64[both], dm[both]
+        ang = torch.atan2(torch.linalg.cross(a, b).norm(dim=1), (a * b).sum(dim=1))
+        acc["angle_max"] = max(acc["angle_max"], float(ang.max()))
+        acc["angle_sum"] += float(ang.sum())
+        acc["escaped"] += int(both.sum())
+        acc["above_1e-3"] += int((ang > 1e-3).sum())
+
+
+def _acc():
+    return {"pixels": 0, "class_mismatch": 0, "escaped": 0, "angle_max": 0.0, "angle_sum": 0.0,
+            "above_1e-3": 0}
+
+
+def _acc_report(a):
+    return {"pixel_class_mismatch_vs_fp64": a["class_mismatch"] / max(1, a["pixels"]),
+            "escape_angle_err_rad": {"max": a["angle_max"],
+                                     "mean": a["angle_sum"] / max(1, a["escaped"]),
+                                     "above_1e-3": a["above_1e-3"], "escaped": a["escaped"]},
+            "pixels": a["pixels"]}
+
+
 def ragged_batch(geon, ctx, torch, stream, ev):
     from geon_amd import workloads
     res = {}
+    outs = {}
     for label, prec in (("fp64", geon.PREC_FP64), ("mixed", geon.PREC_MIXED)):
         geon.autotune(geon.SCHWARZSCHILD, prec, ctx=ctx)
         geon.autotune(geon.ALCUBIERRE, prec, ctx=ctx)
         geon.autotune(geon.MORRIS_THORNE, prec, ctx=ctx)
         frames = workloads.ragged_frames(precision=prec)
         bt = geon.Batch(ctx)
-        ids = [bt.add_frame(m, cam, cfg) for m, cam, cfg in frames]
+        o = []
+        for m, cam, cfg in frames:
+            n = cam.width * cam.height
+            cls = torch.empty(n, dtype=torch.int32, device="cuda")
+            d = torch.empty(3 * n, dtype=torch.float64, device="cuda")
+            o.append((type(m).__name__, cls, d))
+        ids = [bt.add_frame(m, cam, cfg, 0, c.data_ptr(), d.data_ptr())
+               for (m, cam, cfg), (_, c, d) in zip(frames, o)]
         torch.cuda.synchronize()
         a, b = ev(), ev()
         a.record(stream)
@@ -312,17 +383,28 @@ def ragged_batch(geon, ctx, torch, stream, ev):
         steps = sum(int(bt.frame_stats(i)["ray_steps"]) for i in ids)
         tot = bt.totals()
         bt.close()
+        outs[label] = o
         res[label] = {"ms": round(ms, 2), "ray_steps": steps,
                       "ray_steps_per_s": steps / (ms * 1e-3), "fp64_rays": tot["fp64_rays"]}
+    per, allk = {}, _acc()
+    for (kind, c64, d64), (_, cm, dm) in zip(outs["fp64"], outs["mixed"]):
+        _mixed_vs_fp64(torch, c64, d64, cm, dm, per.setdefault(kind, _acc()))
+        _mixed_vs_fp64(torch, c64, d64, cm, dm, allk)
+    res["mixed_vs_fp64"] = _acc_report(allk)
+    res["mixed_vs_fp64_per_metric"] = {k: _acc_report(v) for k, v in per.items()}
     res["workload"] = "256 frames, side log-uniform 64..2048, metrics uniform over " \
                       "{Schwarzschild M∈[0.5,2], Alcubierre v∈[0.1,0.9], Morris-Thorne b0∈[0.5,2]}, " \
                       "escape radius 80, max 5000, seed 1234"
-    res["speedup_mixed_vs_fp64"] = res["mixed"]["ray_steps_per_s"] / res["fp64"]["ray_steps_per_s"]
+    res["speedup_mixed_vs_fp64"] = res["fp64"]["ms"] / res["mixed"]["ms"]
     return res
 
 
 def serving(geon, ctx, torch, stream, outs, ev, seconds):
-    """Poisson request streams of `seconds` of arrivals at each rate."""
+    """Poisson request streams of `seconds` of arrivals at each rate. At each rate
+    the plain FP64 kernel then renders the same requests one after another:
+    `speedup_vs_plain_fp64` is that FP64 device time over the server's wall
+    time for the whole stream (bisections included on the server's side only),
+    and a seeded sample of the served renders is compared with the FP64 frames."""
     import numpy as np
 
     from geon_amd import workloads
@@ -331,9 +413,11 @@ def serving(geon, ctx, torch, stream, outs, ev, seconds):
                        "FP32-mixed), 20% bisections 64 rays × 40 halvings (M=1, r_obs∈[10,50], "
                        "photon-sphere decisions); seed 7; 8 GiB admission budget, 2048-step quanta",
            "rates": {}}
-    plain_rate = None
     for rate in (50, 100, 200, 400):
         reqs = workloads.serving_stream(rate, int(rate * seconds))
+        rng = np.random.default_rng(rate)
+        renders = [i for i, r in enumerate(reqs) if r["kind"] == "render"]
+        keep = set(rng.choice(renders, min(16, len(renders)), replace=False).tolist())
         srv = geon.Server(ctx, hbm_budget=8 << 30, quantum_steps=2048)
         tickets = []
         t_start = time.perf_counter()
@@ -343,7 +427,8 @@ def serving(geon, ctx, torch, stream, outs, ev, seconds):
             while i < len(reqs) and reqs[i]["t"] <= now:
                 r = reqs[i]
                 if r["kind"] == "render":
-                    tickets.append(("render", srv.submit_render(r["metric"], r["camera"], r["cfg"])))
+                    tickets.append(("render", srv.submit_render(r["metric"], r["camera"], r["cfg"],
+                                                                keep=i in keep)))
                 else:
                     tickets.append(("bisect", srv.submit_bisect(r["mass"], r["r_obs"],
                                                                 iters=r["iters"])))
@@ -358,40 +443,50 @@ def serving(geon, ctx, torch, stream, outs, ev, seconds):
             assert res["status"] == 0, res
             lat[kind].append(res["latency_ms"])
             steps += res["ray_steps"]
+        served = {k: srv.fetch_render(tickets[k][1], reqs[k]["camera"].width,
+                                      reqs[k]["camera"].height) for k in sorted(keep)}
         c = srv.counters()
         srv.close()
+        # the same renders through the plain FP64 kernel, one after another
+        fp64_ms, fp64_steps, fp64_gamma, acc = 0.0, 0, 0, _acc()
+        for k, r in enumerate(reqs):
+            if r["kind"] != "render":
+                continue
+            cam = r["camera"]
+            o = outs(cam.width * cam.height)
+            cfg = geon.RenderConfig(step=0.05, max_steps=5000)
+            a, b = ev(), ev()
+            a.record(stream)
+            st = geon.render_dev(r["metric"], cam, cfg, o[0].data_ptr(), o[1].data_ptr(),
+                                 o[2].data_ptr(), stream_ptr=stream.cuda_stream, ctx=ctx)
+            b.record(stream)
+            b.synchronize()
+            fp64_ms += a.elapsed_time(b)
+            fp64_steps += st["ray_steps"]
+            fp64_gamma += st["gamma_steps"]
+            if k in keep:
+                _, cls_m, dir_m = served[k]
+                _mixed_vs_fp64(torch, o[1], o[2], torch.from_numpy(cls_m).cuda(),
+                               torch.from_numpy(np.ascontiguousarray(dir_m.T).reshape(-1)).cuda(),
+                               acc)
         pr = lambda v, q: round(float(np.percentile(v, q)), 2) if v else None  # noqa: E731
         out["rates"][str(rate)] = {
-            "requests": len(tickets), "wall_s": round(wall, 3),
+            "rate": rate, "requests": len(tickets), "wall_s": round(wall, 3),
             "render_latency_ms": {"p50": pr(lat["render"], 50), "p99": pr(lat["render"], 99)},
             "bisect_latency_ms": {"p50": pr(lat["bisect"], 50), "p99": pr(lat["bisect"], 99)},
             "offered_ray_steps_per_s": steps / reqs[-1]["t"],
             "served_ray_steps_per_s": steps / wall, "quanta": c["quanta"],
             "device_busy_frac": round(c["busy_ms"] * 1e-3 / wall, 3),
+            "plain_fp64_same_renders_ms": round(fp64_ms, 2),
+            "plain_fp64_same_renders_ray_steps": fp64_steps,
+            "plain_fp64_same_renders_gamma_ray_steps": fp64_gamma,
+            "speedup_vs_plain_fp64": fp64_ms * 1e-3 / wall,
+            "served_sample_vs_fp64": _acc_report(acc),
         }
         log(f"serving rate {rate}/s: {json.dumps(out['rates'][str(rate)])}")
-        if plain_rate is None:
-            # the same renders through the plain FP64 kernel, one after another
-            tot_steps, tot_ms = 0, 0.0
-            for r in reqs:
-                if r["kind"] != "render":
-                    continue
-                cam = r["camera"]
-                o = outs(cam.width * cam.height)
-                cfg = geon.RenderConfig(step=0.05, max_steps=5000)
-                a, b = ev(), ev()
-                a.record(stream)
-                st = geon.render_dev(r["metric"], cam, cfg, o[0].data_ptr(), o[1].data_ptr(),
-                                     o[2].data_ptr(), stream_ptr=stream.cuda_stream, ctx=ctx)
-                b.record(stream)
-                b.synchronize()
-                tot_ms += a.elapsed_time(b)
-                tot_steps += st["ray_steps"]
-            plain_rate = tot_steps / (tot_ms * 1e-3)
-            out["plain_fp64_same_renders_ray_steps_per_s"] = plain_rate
-    # capacity: the highest sustained rate (arrivals beyond it queue up)
-    best = max(v["served_ray_steps_per_s"] for v in out["rates"].values())
-    out["served_vs_plain_fp64"] = best / plain_rate
+    best = max(out["rates"].values(), key=lambda v: v["speedup_vs_plain_fp64"])
+    out["served_vs_plain_fp64"] = best["speedup_vs_plain_fp64"]
+    out["served_vs_plain_fp64_at_rate"] = best["rate"]
     return out
 
 
@@ -487,10 +582,11 @@ def adaptive(geon, ctx, torch, stream, ev, jobs):
     return res
 
 
-def cpu_baseline(jobs):
+def cpu_baseline(jobs, frame):
     """Oracle (reference algorithm in C) on a bounded sample of the Morris–Thorne rays:
     1024 random pixels on all the threads it may use (~25 CPU-seconds), and 64
-    of them on one thread."""
+    of them on one thread; the sample's classes and escape directions are
+    checked against the GPU frame already in HBM."""
     import platform
 
     import numpy as np
@@ -508,6 +604,14 @@ def cpu_baseline(jobs):
     res = O.trace_rays(om, y0, cfg.step, cfg.max_steps, nthreads=threads)
     steps = int(res["steps"].sum())
     one = O.trace_rays(om, y0[:64], cfg.step, cfg.max_steps, nthreads=1)
+    gcls = frame[1].cpu().numpy()[idx]
+    gdir = frame[2].view(3, -1).T.cpu().numpy()[idx]
+    esc = (gcls == res["cls"]) & (res["cls"] == O.ESCAPED)
+    # rays the reference's fixed step flings out at the coordinate pole end far
+    # past the escape sphere with a direction of amplified rounding noise
+    flung = esc & (np.abs(res["y"][:, 1]) > 10.0 * m.escape_radius)
+    real = esc & ~flung
+    ang = _angle(np, gdir[real], res["dir"][real]) if real.any() else np.zeros(1)
     model = platform.processor() or "unknown"
     try:
         with open("/proc/cpuinfo") as f:
@@ -522,6 +626,9 @@ def cpu_baseline(jobs):
             "sample": f"{len(idx)} random pixels of the Morris-Thorne 1024² render "
                       f"({steps} ray-steps, {res['seconds']:.2f} s on {threads} threads)",
             "single_thread_ray_steps_per_s": int(one["steps"].sum()) / one["seconds"],
+            "gpu_frame_check": {"class_mismatches": int((gcls != res["cls"]).sum()),
+                                "escaped": int(real.sum()), "flung_at_pole": int(flung.sum()),
+                                "max_angle_rad": float(ang.max())},
             "host_cpus": os.cpu_count(), "cpu_model": model}
 
 
