@@ -88,6 +88,7 @@ SIGNATURES = [
     ("geon_metric_terminate", _i, [C.POINTER(Metric), _vp, C.POINTER(C.c_int32)]),
     ("geon_metric_asymptotic_direction", _i, [C.POINTER(Metric), _vp, _vp, _vp]),
     ("geon_camera_new", _i, [_vp, C.c_double, _u32, _u32, C.POINTER(Camera)]),
+    ("geon_sky_sample", _i, [_vp, C.c_int32, _vp]),
     ("geon_ctx_create", _i, [_i, C.c_char_p, C.POINTER(_vp)]),
     ("geon_ctx_destroy", None, [_vp]),
     ("geon_tetrad", _i, [C.POINTER(Metric), _vp, _vp]),

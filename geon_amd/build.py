@@ -6,6 +6,7 @@ Outputs (git-ignored, shipped beside the sources):
   geon_amd/lib/geon_kernels.hsaco   hipcc --genco --offload-arch=gfx950
   geon_amd/lib/libgeon_hip.so       host runtime + C ABI (g++, links libamdhip64)
   geon_amd/lib/opcount              FP64 operation counter of the device code
+  geon_amd/lib/geon-render          the render command line (bin/render.rs flags)
 
 `python -m geon_amd.build --opcount` also rewrites geon_amd/opcount/op_counts.json,
 the per-ray-step and per-point operation counts used for the roofline.
@@ -68,6 +69,21 @@ def build_host(out=None):
     return out
 
 
+CLI_SRC = os.path.join(HERE, "cpp", "render_main.cpp")
+
+
+def build_cli(out=None):
+    """geon-render: the reference's render binary over the C ABI (links libgeon_hip.so)."""
+    out = out or os.path.join(LIB, "geon-render")
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    tmp = out + ".tmp"
+    cxx = shutil.which("g++") or "g++"
+    _run([cxx, "-O2", "-std=c++17", "-Wall", "-Wno-unknown-pragmas", CLI_SRC, f"-L{LIB}",
+          "-lgeon_hip", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{ROCM}/lib", "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
 OPCOUNT_SRC = os.path.join(HERE, "opcount", "opcount.cpp")
 OPCOUNT_JSON = os.path.join(HERE, "opcount", "op_counts.json")
 
@@ -95,6 +111,7 @@ def run_opcount(write=False):
 def build_all():
     build_kernels()
     build_host()
+    build_cli()
 
 
 if __name__ == "__main__":

@@ -18,6 +18,7 @@
 //                  zero for Minkowski (minkowski.rs:28-30); otherwise the
 //                  trait default from g⁻¹ and ∂g (mod.rs:51-69)
 //   terminate, asymptotic_direction   each metric's own rule
+// and sky::sample (sky.rs:107-120), the kernels' sky_sample.
 #include <string.h>
 
 #include "geon_runtime.hpp"
@@ -139,6 +140,12 @@ int geon_metric_asymptotic_direction(const geon_metric_t* metric, const double x
         geon::direction<MP>(y, dir);
         return GEON_OK;
     });
+}
+
+int geon_sky_sample(const double dir[3], int32_t show_stars, float rgb[3]) {
+    if (!dir || !rgb) return fail(GEON_E_INVALID, "bad argument");
+    geon::sky_sample(dir, show_stars != 0, rgb);
+    return GEON_OK;
 }
 
 }  // extern "C"

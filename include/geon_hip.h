@@ -31,6 +31,7 @@
  *   geon_shadow_bisect       bin/validate_shadow.rs:20-95 (is_captured +
  *                            bisection on the launch angle)
  *   geon_tonemap_u8          render::write_png tonemap (src/render.rs:169-180)
+ *   geon_sky_sample          sky::sample               (src/sky.rs:107-120)
  *
  * Memory: "_dev" entry points take device pointers (HBM) and a hipStream_t
  * passed as void* (NULL: the context's own stream); they enqueue work and
@@ -186,6 +187,10 @@ int  geon_metric_terminate(const geon_metric_t* metric, const double x[4], int32
  * morris_thorne.rs:54-77): unit Cartesian direction of an escaped ray. */
 int  geon_metric_asymptotic_direction(const geon_metric_t* metric, const double x[4],
                                       const double p[4], double dir_out[3]);
+
+/* sky::sample (sky.rs:107-120) on the host: linear RGB of the procedural sky
+ * in the unit Cartesian direction dir, stars on or off (sky.rs:23). */
+int  geon_sky_sample(const double dir[3], int32_t show_stars, float rgb_out[3]);
 
 /* Camera::new (camera.rs:29-36): fov_y_rad = fov_y_deg·(π/180), as
  * f64::to_radians computes it. */

@@ -140,6 +140,14 @@ def _x4(x):
     return np.ascontiguousarray(x, dtype=np.float64)
 
 
+def dot4(gm, a, b):
+    """types.rs:22-31."""
+    g16 = np.ascontiguousarray(gm, dtype=np.float64).reshape(16)
+    L = lib()
+    L.or_dot4.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    return L.or_dot4(g16.ctypes.data, _x4(a).ctypes.data, _x4(b).ctypes.data)
+
+
 def g(m, x):
     out = np.zeros(16)
     lib().or_g(C.byref(m), _dp(_x4(x)), _dp(out))
