@@ -101,6 +101,10 @@ SIGNATURES = [
                          _vp, _vp, _vp, C.POINTER(RenderStats)]),
     ("geon_render_dev", _i, [_vp, C.POINTER(Metric), C.POINTER(Camera), C.POINTER(RenderCfg),
                              _vp, _vp, _vp, C.POINTER(RenderStats), _vp]),
+    ("geon_render_dev_noalloc", _i, [_vp, C.POINTER(Metric), C.POINTER(Camera),
+                                     C.POINTER(RenderCfg), _vp, _vp, _vp, C.POINTER(RenderStats),
+                                     _vp]),
+    ("geon_render_dev_noalloc_max_rays", C.c_uint64, []),
     ("geon_curvature", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_int32, _vp]),
     ("geon_curvature_dev", _i, [_vp, C.POINTER(Metric), _vp, _sz, C.c_int32, _vp, _vp]),
     ("geon_nec_contraction", _i, [_vp, C.POINTER(Metric), _vp, _sz, _vp, _vp]),

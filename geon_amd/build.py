@@ -4,7 +4,8 @@
 
 Outputs (git-ignored, shipped beside the sources):
   geon_amd/lib/geon_kernels.hsaco   hipcc --genco --offload-arch=gfx950
-  geon_amd/lib/libgeon_hip.so       host runtime + C ABI (g++, links libamdhip64)
+  geon_amd/lib/libgeon_hip.so       host runtime + C ABI (g++, links libamdhip64), plus
+                                    geon_noalloc.hip (hipcc -c: a device array of its own)
   geon_amd/lib/opcount              FP64 operation counter of the device code
   geon_amd/lib/geon-render          the render command line (bin/render.rs flags)
 
@@ -17,6 +18,7 @@ import os
 import shutil
 import subprocess
 import sys
+import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -41,6 +43,13 @@ KERNEL_DEPS = KERNEL_SRC + [os.path.join(CSRC, "kernels", f)
 HOST_SRC = [os.path.join(CSRC, "host", f)
             for f in ("geon_runtime.cpp", "geon_abi.cpp", "geon_batch.cpp", "geon_serve.cpp",
                       "geon_metric_host.cpp")]
+# The allocation-free render keeps its scratch in a file-scope __device__
+# array, so its translation unit is compiled by hipcc (host code plus the
+# gfx950 image that holds the array) and linked into the same library.
+NOALLOC_SRC = os.path.join(CSRC, "host", "geon_noalloc.hip")
+NOALLOC_FLAGS = ["-c", "-fPIC", "--offload-arch=gfx950", "-O2", "-std=c++17", "-fvisibility=hidden",
+                 "-ffp-contract=off", "-Wall", "-Xclang", "-target-feature", "-Xclang",
+                 "-packed-fp32-ops"]
 
 
 def _run(cmd):
@@ -62,9 +71,12 @@ def build_host(out=None):
     os.makedirs(os.path.dirname(out), exist_ok=True)
     tmp = out + ".tmp"
     cxx = shutil.which("g++") or "g++"
-    _run([cxx, *HOST_FLAGS, *HOST_SRC, f"-L{ROCM}/lib", "-lamdhip64", "-lpthread", "-lm",
-          f"-Wl,-rpath,{ROCM}/lib",
-          "-o", tmp])
+    with tempfile.TemporaryDirectory() as td:
+        obj = os.path.join(td, "geon_noalloc.o")
+        _run([HIPCC, *NOALLOC_FLAGS, NOALLOC_SRC, "-o", obj])
+        _run([cxx, *HOST_FLAGS, *HOST_SRC, obj, f"-L{ROCM}/lib", "-lamdhip64", "-lpthread", "-lm",
+              f"-Wl,-rpath,{ROCM}/lib",
+              "-o", tmp])
     os.replace(tmp, out)
     return out
 

@@ -39,6 +39,72 @@ int fill_camera(const geon_metric_t* metric, const geon_camera_t* cam, geon_dfra
 
 }  // namespace
 
+namespace geon_host {
+
+int fp64_frame(const geon_metric_t* metric, const geon_camera_t* cam, const geon_render_cfg_t* cfg,
+               geon_dframe* F, uint64_t* n_pix, uint64_t* n_rays) {
+    GEON_TRY(fill_frame(metric, F));
+    GEON_TRY(fill_camera(metric, cam, F));
+    const uint32_t spa = cfg->samples_per_axis ? cfg->samples_per_axis : 1;
+    F->spa = spa;
+    F->inv_spa = 1.0 / (double)spa;
+    F->step = cfg->step;
+    F->max_steps = cfg->max_steps;
+    memcpy(F->captured_rgb, cfg->captured_rgb, sizeof F->captured_rgb);
+    memcpy(F->diverged_rgb, cfg->diverged_rgb, sizeof F->diverged_rgb);
+    F->show_stars = cfg->show_stars;
+    F->precision = GEON_PREC_FP64;
+    *n_pix = (uint64_t)cam->width * cam->height;
+    *n_rays = *n_pix * spa * spa;
+    return GEON_OK;
+}
+
+int fp64_run(geon_ctx* ctx, const geon_dframe* F, uint64_t n_pix, uint64_t n_rays,
+             const fp64_planes& P, float* rgb, int32_t* cls_per_pixel, double* dir_per_pixel,
+             geon_render_stats_t* stats, hipStream_t s, hipEvent_t t0, hipEvent_t t1) {
+    uint32_t slot;
+    GEON_TRY(ctx_slot(ctx, F, s, &slot));
+    geon_dstats* d_stats = P.stats;
+    int32_t* d_cls = P.cls;
+    uint32_t* d_steps = P.steps;
+    double* d_res = P.res;
+    double* d_dir = P.dir;
+    void* cargs[] = {&d_stats};
+    GEON_HIP(hipModuleLaunchKernel(ctx->fn_stats_clear, 1, 1, 1, 64, 1, 1, 0, s, cargs, nullptr),
+             "stats clear");
+    if (t0) GEON_HIP(hipEventRecord(t0, s), "event");
+    void* rargs[] = {&slot, (void*)&n_rays, &d_cls, &d_steps, &d_res, &d_dir, &d_stats};
+    GEON_TRY(launch(ctx->fn_render[F->m.kind], n_rays, 256, s, rargs));
+    if (t1) GEON_HIP(hipEventRecord(t1, s), "event");
+    void* fargs[] = {&slot, (void*)&n_pix, &d_cls, &d_steps, &d_res, &d_dir,
+                     &rgb, &cls_per_pixel, &dir_per_pixel, &d_stats};
+    GEON_TRY(launch(ctx->fn_finalize, n_pix, 256, s, fargs));
+    GEON_TRY(ctx_slot_done(ctx, slot, s));
+    if (stats) {
+        geon_dstats ds;
+        GEON_HIP(hipMemcpyDtoHAsync(&ds, (hipDeviceptr_t)d_stats, sizeof ds, s), "D2H stats");
+        GEON_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
+        float ms = 0.0f;
+        if (t0 && t1) hipEventElapsedTime(&ms, t0, t1);
+        memset(stats, 0, sizeof *stats);
+        stats->captured = ds.captured;
+        stats->escaped = ds.escaped;
+        stats->diverged = ds.diverged;
+        double mr;
+        memcpy(&mr, &ds.max_resid_bits, sizeof mr);
+        stats->max_null_residual = mr;
+        stats->avg_steps = (double)ds.steps_pix_sum / (double)n_pix;
+        stats->max_steps_hit = ds.max_steps_hit;
+        stats->ray_steps = ds.ray_steps;
+        stats->fp64_rays = n_rays;
+        stats->kernel_ms = ms;
+        stats->gamma_steps = ds.gamma_steps;
+    }
+    return GEON_OK;
+}
+
+}  // namespace geon_host
+
 // Only the C ABI is exported; the library is built with -fvisibility=hidden.
 #pragma GCC visibility push(default)
 extern "C" {
@@ -255,59 +321,21 @@ int geon_render_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_cam
     if (cfg->precision != GEON_PREC_FP64)
         return fail(GEON_E_INVALID, "unknown precision %d", cfg->precision);
     geon_dframe F;
-    GEON_TRY(fill_frame(metric, &F));
-    GEON_TRY(fill_camera(metric, cam, &F));
-    const uint32_t spa = cfg->samples_per_axis ? cfg->samples_per_axis : 1;
-    F.spa = spa;
-    F.inv_spa = 1.0 / (double)spa;
-    F.step = cfg->step;
-    F.max_steps = cfg->max_steps;
-    memcpy(F.captured_rgb, cfg->captured_rgb, sizeof F.captured_rgb);
-    memcpy(F.diverged_rgb, cfg->diverged_rgb, sizeof F.diverged_rgb);
-    F.show_stars = cfg->show_stars;
-    F.precision = cfg->precision;
-    const uint64_t n_pix = (uint64_t)cam->width * cam->height;
-    const uint64_t n_rays = n_pix * spa * spa;
+    uint64_t n_pix, n_rays;
+    GEON_TRY(fp64_frame(metric, cam, cfg, &F, &n_pix, &n_rays));
+    fp64_planes P;
     void *d_cls, *d_steps, *d_res, *d_dir;
     GEON_TRY(lane_buffer(lane, "ray_cls", n_rays * 4, &d_cls));
     GEON_TRY(lane_buffer(lane, "ray_steps", n_rays * 4, &d_steps));
     GEON_TRY(lane_buffer(lane, "ray_res", n_rays * 8, &d_res));
     GEON_TRY(lane_buffer(lane, "ray_dir", n_rays * 24, &d_dir));
-    uint32_t slot;
-    GEON_TRY(ctx_slot(ctx, &F, s, &slot));
-    geon_dstats* d_stats = lane->d_stats;
-    void* cargs[] = {&d_stats};
-    GEON_HIP(hipModuleLaunchKernel(ctx->fn_stats_clear, 1, 1, 1, 64, 1, 1, 0, s, cargs, nullptr),
-             "stats clear");
-    GEON_HIP(hipEventRecord(lane->t0, s), "event");
-    void* rargs[] = {&slot, (void*)&n_rays, &d_cls, &d_steps, &d_res, &d_dir, &d_stats};
-    GEON_TRY(launch(ctx->fn_render[metric->kind], n_rays, 256, s, rargs));
-    GEON_HIP(hipEventRecord(lane->t1, s), "event");
-    void* fargs[] = {&slot, (void*)&n_pix, &d_cls, &d_steps, &d_res, &d_dir,
-                     &rgb, &cls_per_pixel, &dir_per_pixel, &d_stats};
-    GEON_TRY(launch(ctx->fn_finalize, n_pix, 256, s, fargs));
-    GEON_TRY(ctx_slot_done(ctx, slot, s));
-    if (stats) {
-        geon_dstats ds;
-        GEON_HIP(hipMemcpyDtoHAsync(&ds, (hipDeviceptr_t)d_stats, sizeof ds, s), "D2H stats");
-        GEON_HIP(hipStreamSynchronize(s), "hipStreamSynchronize");
-        float ms = 0.0f;
-        hipEventElapsedTime(&ms, lane->t0, lane->t1);
-        memset(stats, 0, sizeof *stats);
-        stats->captured = ds.captured;
-        stats->escaped = ds.escaped;
-        stats->diverged = ds.diverged;
-        double mr;
-        memcpy(&mr, &ds.max_resid_bits, sizeof mr);
-        stats->max_null_residual = mr;
-        stats->avg_steps = (double)ds.steps_pix_sum / (double)n_pix;
-        stats->max_steps_hit = ds.max_steps_hit;
-        stats->ray_steps = ds.ray_steps;
-        stats->fp64_rays = n_rays;
-        stats->kernel_ms = ms;
-        stats->gamma_steps = ds.gamma_steps;
-    }
-    return GEON_OK;
+    P.cls = (int32_t*)d_cls;
+    P.steps = (uint32_t*)d_steps;
+    P.res = (double*)d_res;
+    P.dir = (double*)d_dir;
+    P.stats = lane->d_stats;
+    return fp64_run(ctx, &F, n_pix, n_rays, P, rgb, cls_per_pixel, dir_per_pixel, stats, s, lane->t0,
+                    lane->t1);
 }
 
 int geon_render(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_camera_t* cam,

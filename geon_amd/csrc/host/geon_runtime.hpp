@@ -84,6 +84,26 @@ int launch(hipFunction_t f, uint64_t n_threads, uint32_t block, hipStream_t stre
 int frame_from(const geon_metric_t* metric, const geon_camera_t* cam, const geon_render_cfg_t* cfg,
                geon_dframe* F);
 
+// The per-ray planes of one FP64 render (n_rays entries each) and its
+// statistics block: lane buffers for geon_render_dev, the fixed device array
+// for geon_render_dev_noalloc.
+struct fp64_planes {
+    int32_t* cls = nullptr;
+    uint32_t* steps = nullptr;
+    double* res = nullptr;
+    double* dir = nullptr;
+    geon_dstats* stats = nullptr;
+};
+// The constant frame of an FP64 render and its pixel and ray counts.
+int fp64_frame(const geon_metric_t* metric, const geon_camera_t* cam, const geon_render_cfg_t* cfg,
+               geon_dframe* F, uint64_t* n_pix, uint64_t* n_rays);
+// Enqueues the FP64 render on `s` (statistics clear, ray kernel, pixel
+// finalize); with `stats` it waits for the stream and fills them. t0/t1,
+// when given, bracket the ray kernel for stats->kernel_ms.
+int fp64_run(geon_ctx* ctx, const geon_dframe* F, uint64_t n_pix, uint64_t n_rays,
+             const fp64_planes& P, float* rgb, int32_t* cls_per_pixel, double* dir_per_pixel,
+             geon_render_stats_t* stats, hipStream_t s, hipEvent_t t0, hipEvent_t t1);
+
 }  // namespace geon_host
 
 #define GEON_TRY(expr)                 \

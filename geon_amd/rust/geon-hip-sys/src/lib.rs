@@ -182,6 +182,11 @@ extern "C" {
                            cfg: *const geon_render_cfg_t, rgb: *mut f32, cls_per_pixel: *mut i32,
                            dir_per_pixel: *mut f64, stats: *mut geon_render_stats_t,
                            stream: *mut c_void) -> c_int;
+    pub fn geon_render_dev_noalloc(ctx: *mut geon_ctx_t, m: *const geon_metric_t,
+                                   cam: *const geon_camera_t, cfg: *const geon_render_cfg_t,
+                                   rgb: *mut f32, cls_per_pixel: *mut i32, dir_per_pixel: *mut f64,
+                                   stats: *mut geon_render_stats_t, stream: *mut c_void) -> c_int;
+    pub fn geon_render_dev_noalloc_max_rays() -> u64;
 
     pub fn geon_curvature(ctx: *mut geon_ctx_t, m: *const geon_metric_t, x_soa: *const f64,
                           n: usize, what: i32, out: *mut f64) -> c_int;

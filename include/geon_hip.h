@@ -241,6 +241,21 @@ int  geon_render(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_camera
 int  geon_render_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const geon_camera_t* cam,
                      const geon_render_cfg_t* cfg, float* rgb, int32_t* cls_per_pixel,
                      double* dir_per_pixel, geon_render_stats_t* stats, void* stream);
+/* geon_render_dev without any allocation (no counterpart in the reference:
+ * for callers that cannot allocate — code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). FP64 precision only
+ * (GEON_E_UNSUPPORTED otherwise); at most geon_render_dev_noalloc_max_rays()
+ * rays = W·H·samples² per call (4,194,304, e.g. one 2048² frame; larger:
+ * GEON_E_INVALID). The per-ray planes and statistics live in one fixed
+ * device array per GPU (168 MiB, held until the process ends) shared by
+ * every call on that GPU: calls that overlap on one GPU race, so order them
+ * on one stream. Results are those of geon_render_dev; stats->kernel_ms is
+ * 0 (no events). */
+int  geon_render_dev_noalloc(geon_ctx_t* ctx, const geon_metric_t* metric,
+                             const geon_camera_t* cam, const geon_render_cfg_t* cfg, float* rgb,
+                             int32_t* cls_per_pixel, double* dir_per_pixel,
+                             geon_render_stats_t* stats, void* stream);
+uint64_t geon_render_dev_noalloc_max_rays(void);
 
 /* geon_trace_rays with adaptive steps: Cash–Karp RK4(5), a step accepted
  * when max_i |err_i| / (atol + rtol·max(|y_i|, |y_new_i|)) ≤ 1, next step
