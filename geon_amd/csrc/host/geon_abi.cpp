@@ -380,8 +380,11 @@ int geon_curvature_dev(geon_ctx_t* ctx, const geon_metric_t* metric, const doubl
         GEON_TRY(lane_buffer(lane, "curv_scratch", n * 256 * 8, &scratch));
     }
     void* args[] = {&slot, &nn, (void*)&x_soa, &what, &out, &scratch, &factor};
-    if (metric->kind == GEON_METRIC_ALCUBIERRE)   // one 8-lane team per point, 8 points per block
+    if (metric->kind == GEON_METRIC_ALCUBIERRE && what == GEON_CURV_RIEMANN)
+        // one 8-lane team per point, 8 points per block
         GEON_TRY(launch(ctx->fn_curv[metric->kind], ((nn + 7) / 8) * 64, 64, s, args));
+    else if (metric->kind == GEON_METRIC_ALCUBIERRE)   // one quad per point, 16 points per block
+        GEON_TRY(launch(ctx->fn_curv_alc_ricci, ((nn + 15) / 16) * 64, 64, s, args));
     else                                          // one lane per point
         GEON_TRY(launch(ctx->fn_curv[metric->kind], nn, 256, s, args));
     return ctx_slot_done(ctx, slot, s);

@@ -211,6 +211,7 @@ int ctx_create(int device, const char* code_object, geon_ctx** out) {
     ok = ok && get(&c->fn_stats_clear, "geon_stats_clear");
     ok = ok && get(&c->fn_bisect, "geon_bisect_serial");
     ok = ok && get(&c->fn_nec, "geon_nec_kk");
+    ok = ok && get(&c->fn_curv_alc_ricci, "geon_curvature_alc_ricci");
     ok = ok && get(&c->fn_job_finalize, "geon_job_finalize");
     ok = ok && get(&c->fn_zero, "geon_zero_u64");
     ok = ok && get(&c->fn_fill, "geon_fill_u64");

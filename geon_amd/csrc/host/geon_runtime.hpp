@@ -38,6 +38,7 @@ struct geon_ctx {
     hipFunction_t fn_stats_clear = nullptr;
     hipFunction_t fn_bisect = nullptr;
     hipFunction_t fn_nec = nullptr;
+    hipFunction_t fn_curv_alc_ricci = nullptr;   // Alcubierre Ricci/R/G/T by quads
     hipFunction_t fn_sched[4][2][2] = {};   // [kind][0 f64 | 1 f32][rays_per_lane - 1]
     hipFunction_t fn_job_finalize = nullptr;
     hipFunction_t fn_zero = nullptr;

@@ -448,7 +448,9 @@ GEON_CURV_KERNEL(mt, MorrisThorne)
 // metric points: a team of 8 lanes per point evaluates the 31 distinct
 // metric points once, forms the seven Γ on seven lanes and the outputs on
 // all eight, exchanging through LDS (geon_curv.hpp); 64-lane blocks,
-// 8 points per block. Same outputs bit for bit as one lane per point.
+// 8 points per block. Same outputs bit for bit as one lane per point. The
+// host launches it for Riemann (what 0); the Ricci family goes to
+// geon_curvature_alc_ricci below, which gives the same values.
 extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc(
     uint32_t slot, uint64_t n, const double* xs, int32_t what, double* out, double* scratch,
     double factor) {
@@ -507,6 +509,97 @@ extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc(
     }
     out[k * 16 + t] = e[0];
     out[k * 16 + t + GEON_CURV_TEAM] = e[1];
+}
+
+// A double moved between the lanes of each quad by DPP quad_perm: CTRL =
+// 0xB1 swaps lanes 0↔1 and 2↔3, k·0x55 broadcasts lane k. Every lane of
+// the wave must be active.
+template <int CTRL>
+__device__ double quad_dpp(double v) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);
+    lo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xF, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                            (unsigned long long)(unsigned)lo));
+}
+
+// Ricci (what 1), Einstein (2), stress-energy (3) and the Ricci scalar (4) of
+// Alcubierre points by quads (AlcQuad in geon_curv.hpp): 4 lanes per point,
+// 16 points per 64-lane block; the same outputs bit for bit as
+// geon_curvature_alc, which keeps the Riemann output (what 0). Every lane
+// runs every phase (a point past n computes at x = 0 and stores nothing), so
+// that the DPP exchanges see the whole wave.
+#define GEON_CURV_QPTS 16
+extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc_ricci(
+    uint32_t slot, uint64_t n, const double* xs, int32_t what, double* out, double* scratch,
+    double factor) {
+    (void)scratch;
+    __shared__ double s_pt[GEON_CURV_QPTS][AlcQuad::stride()];
+    const geon_dmetric& P = c_frames[slot].m;
+    const int t = (int)(threadIdx.x & 3u);
+    const int q = (int)(threadIdx.x >> 2);
+    const uint64_t k = (uint64_t)blockIdx.x * GEON_CURV_QPTS + q;
+    const bool valid = k < n;
+    double x[4] = {0.0, 0.0, 0.0, 0.0};
+    if (valid) {
+        x[0] = xs[k];
+        x[1] = xs[n + k];
+        x[2] = xs[2 * n + k];
+        x[3] = xs[3 * n + k];
+    }
+    double* pt = s_pt[q];
+    // A: the 31 distinct metric points, 8 rounds of the 4 lanes
+    double sq[3];
+    AlcQuad::base_squares(P, x, sq);
+    sfor<8>([&](auto R) { AlcQuad::shape_round<CI(R)>(P, x, sq, t, pt); });
+    __syncthreads();
+    // B: Γ at x ± h e_x, x ± h e_y (Γ-stencil points 1-4), then x ± h e_z
+    // (5, 6) on lanes 0-1 and x (0) on lanes 2-3
+    double ga[30], gb[30], oa[30], ob[30];
+    curv_alc_gamma(P, pt, 1 + t, ga);
+    curv_alc_gamma(P, pt, t < 2 ? 5 + t : 0, gb);
+    const double f0 = pt[0];
+    sfor<64>([&](auto I) {
+        constexpr int i = CI(I), mu = i / 16, a = (i / 4) % 4, b = i % 4;
+        if constexpr (AlcQuad::wanted(0x6, mu, a, b)) {
+            constexpr int sl = AlcQuad::slot(mu, a, b);
+            oa[sl] = quad_dpp<0xB1>(ga[sl]);
+        }
+        if constexpr (AlcQuad::wanted(0x8, mu, a, b)) {
+            constexpr int sl = AlcQuad::slot(mu, a, b);
+            ob[sl] = quad_dpp<0xB1>(gb[sl]);
+        }
+    });
+    __syncthreads();   // every shape value has been read: the rows go over them
+    const double ih = P.fd_inv_2h;
+    if (t == 0) AlcQuad::drow<1>(ga, oa, ih, pt);   // lane 0 holds x + h e_x
+    if (t == 1) AlcQuad::drow<3>(ob, gb, ih, pt);   // lane 1 holds x − h e_z
+    if (t == 2) AlcQuad::drow<2>(ga, oa, ih, pt);   // lane 2 holds x + h e_y
+    if (t == 3) AlcQuad::row0(gb, pt);              // lane 3 holds Γ(x)
+    __syncthreads();
+    // C: R_σν of lane σ
+    double ric[4];
+    AlcQuad::ricci(pt, t, ric);
+    if (what == 1) {
+        if (valid) {
+#pragma unroll
+            for (int nu = 0; nu < 4; ++nu) out[k * 16 + t * 4 + nu] = ric[nu];
+        }
+        return;
+    }
+    // D: R from the quad, then G or T
+    const double r6[6] = {quad_dpp<0x00>(ric[0]), quad_dpp<0x00>(ric[1]), quad_dpp<0x55>(ric[0]),
+                          quad_dpp<0x55>(ric[1]), quad_dpp<0xAA>(ric[2]), quad_dpp<0xFF>(ric[3])};
+    double e[4], R;
+    AlcQuad::outputs(P, f0, r6, ric, t, what, factor, e, &R);
+    if (!valid) return;
+    if (what == 4) {
+        if (t == 0) out[k] = R;
+        return;
+    }
+#pragma unroll
+    for (int nu = 0; nu < 4; ++nu) out[k * 16 + t * 4 + nu] = e[nu];
 }
 
 // ---- shadow bisection (validate_shadow.rs:20-95) ----------------------------

@@ -96,3 +96,54 @@ extern "C" int hd_curvature_alc_team(const geon_metric_t* m, const double* x, lo
     }
     return 0;
 }
+
+// The quad form of the Ricci family (AlcQuad): the four lanes of a point
+// run one after another with the point's LDS row; what as above, 1-4 only.
+extern "C" int hd_curvature_alc_quad(const geon_metric_t* m, const double* x, long n, int what,
+                                     double factor, double* out) {
+    geon_dmetric P;
+    if (m->kind != GEON_METRIC_ALCUBIERRE || what < 1 || what > 4 ||
+        make_dmetric(m, &P) != GEON_OK)
+        return -100;
+    for (long k = 0; k < n; ++k) {
+        const double* xk = x + 4 * k;
+        double pt[AlcQuad::stride()];
+        for (int i = 0; i < AlcQuad::stride(); ++i) pt[i] = -1.0;   // unread slots show up
+        double sq[3];
+        AlcQuad::base_squares(P, xk, sq);
+        for (int t = 0; t < 4; ++t) {
+            AlcQuad::shape_round<0>(P, xk, sq, t, pt);
+            AlcQuad::shape_round<1>(P, xk, sq, t, pt);
+            AlcQuad::shape_round<2>(P, xk, sq, t, pt);
+            AlcQuad::shape_round<3>(P, xk, sq, t, pt);
+            AlcQuad::shape_round<4>(P, xk, sq, t, pt);
+            AlcQuad::shape_round<5>(P, xk, sq, t, pt);
+            AlcQuad::shape_round<6>(P, xk, sq, t, pt);
+            AlcQuad::shape_round<7>(P, xk, sq, t, pt);
+        }
+        double gs[7][30];
+        for (int t = 0; t < 7; ++t) curv_alc_gamma(P, pt, t, gs[t]);
+        const double f0 = pt[0];
+        AlcQuad::drow<1>(gs[1], gs[2], P.fd_inv_2h, pt);
+        AlcQuad::drow<2>(gs[3], gs[4], P.fd_inv_2h, pt);
+        AlcQuad::drow<3>(gs[5], gs[6], P.fd_inv_2h, pt);
+        AlcQuad::row0(gs[0], pt);
+        double ric[4][4];
+        for (int sg = 0; sg < 4; ++sg) AlcQuad::ricci(pt, sg, ric[sg]);
+        if (what == 1) {
+            for (int i = 0; i < 16; ++i) out[16 * k + i] = ric[i / 4][i % 4];
+            continue;
+        }
+        const double r6[6] = {ric[0][0], ric[0][1], ric[1][0], ric[1][1], ric[2][2], ric[3][3]};
+        for (int sg = 0; sg < 4; ++sg) {
+            double e[4], R;
+            AlcQuad::outputs(P, f0, r6, ric[sg], sg, what, factor, e, &R);
+            if (what == 4) {
+                if (sg == 0) out[k] = R;
+                continue;
+            }
+            for (int nu = 0; nu < 4; ++nu) out[16 * k + 4 * sg + nu] = e[nu];
+        }
+    }
+    return 0;
+}

@@ -149,3 +149,10 @@ def test_alcubierre_curvature_team_matches_oracle(hd, O):
                                             C.c_double(factor), _p(got)) == 0
             ref, _ = O.curvature(m, x, what)
             assert np.array_equal(got, np.asarray(ref).reshape(-1)), what
+            if what == O.CURV_RIEMANN:
+                continue
+            # the quad form of the Ricci family
+            got = np.zeros(len(x) * width)
+            assert hd.hd_curvature_alc_quad(C.byref(m), _p(x), C.c_long(len(x)), what,
+                                            C.c_double(factor), _p(got)) == 0
+            assert np.array_equal(got, np.asarray(ref).reshape(-1)), ("quad", what)

@@ -221,6 +221,30 @@ def test_c2_curvature_full_launches(ctx, O):
     assert np.abs(Tm[:, 0, 0] - closed).max() <= 1e-6
 
 
+def test_alcubierre_ricci_quads_match_riemann_team(ctx):
+    """The Ricci family of Alcubierre runs on quads (4 lanes per point) and
+    Riemann on 8-lane teams. R_σν = Σ_α R^α_σαν (curvature.rs:72-85) formed
+    on the host from the team kernel's Riemann, summed from 0.0 over α in
+    order, equals the quad kernel's Ricci bit for bit: both form every
+    R^α_σαν by the same operations. Points: a seeded sample of the 128³ grid
+    plus the wall, the centre and the flat region, for two bubbles."""
+    import geon_amd as geon
+    rng = np.random.default_rng(11)
+    g = -4.0 + 8.0 * np.arange(128) / 127.0
+    idx = rng.integers(0, 128, (3000, 3))
+    pts = np.c_[np.zeros(len(idx)), g[idx[:, 0]], g[idx[:, 1]], g[idx[:, 2]]]
+    pts = np.r_[pts, [[0, 2.0, 0, 0], [0, 0, 0, 0], [0, -1.9, 0.3, 0.1], [0, 6.0, 1.0, -2.0]]]
+    for m in (geon.Alcubierre(bubble_radius=2.0, bubble_thickness=0.125, ship_velocity=0.5),
+              geon.Alcubierre()):
+        riem = geon.riemann(m, pts, ctx=ctx).reshape(len(pts), 4, 4, 4, 4)
+        ric = geon.ricci(m, pts, ctx=ctx).reshape(len(pts), 4, 4)
+        want = np.zeros((len(pts), 4, 4))
+        for a in range(4):
+            want = want + riem[:, a, :, a, :]
+        assert np.array_equal(ric.view(np.uint64), want.view(np.uint64))
+        assert np.abs(ric).max() > 0.0
+
+
 def test_alcubierre_long_flat_runs(ctx, O):
     """A narrow camera 40 units out on the bubble's axis: every ray first
     crosses ~35 units of exactly flat space (flatness tested once per run of
