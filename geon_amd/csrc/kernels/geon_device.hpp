@@ -194,7 +194,7 @@ GEON_D double tanh_cw(double x) {
 }
 // The same as a call: the fourteen tanh of an Alcubierre Γ, inlined and
 // interleaved, would push the ray kernels past 256 VGPRs.
-__device__ __attribute__((noinline)) double tanh_cw_call(double x) { return tanh_cw(x); }
+__device__ __attribute__((noinline)) inline double tanh_cw_call(double x) { return tanh_cw(x); }
 
 // sin and cos of θ = θ0 + d from s0 = sin θ0 and c0 = cos θ0, for
 // |d| ≤ 1/16 by rotation: sin d = d + d³·P(d²) and cos d − 1 = d²·Q(d²)
@@ -1123,18 +1123,18 @@ GEON_D int trace_adaptive(const geon_dmetric& P, double* y, double h0, double rt
             const double sc = atol + rtol * fmax(fabs(y[i]), fabs(yn[i]));
             e = fmax(e, fabs(err[i]) / sc);
         }
-        if (e > 1.0) {
+        // a trial step that is not finite (fmax above drops a NaN from e) is
+        // rejected like one that is too long, with the smallest factor 0.1;
+        // the ray diverges only when h falls below h_min
+        const bool bad = !finite8(yn) || !finite8(err);
+        if (bad || e > 1.0) {
             ++rej;
-            h = h * fmax(0.9 / sqrt(sqrt(e)), 0.1);
+            h = h * (bad ? 0.1 : fmax(0.9 / sqrt(sqrt(e)), 0.1));
             if (!(fabs(h) >= h_min)) {
                 cls = DIVERGED;
                 break;
             }
             continue;
-        }
-        if (!finite8(yn)) {
-            cls = DIVERGED;
-            break;
         }
 #pragma unroll
         for (int i = 0; i < 8; ++i) y[i] = yn[i];

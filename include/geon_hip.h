@@ -260,6 +260,7 @@ uint64_t geon_render_dev_noalloc_max_rays(void);
 /* geon_trace_rays with adaptive steps: Cash–Karp RK4(5), a step accepted
  * when max_i |err_i| / (atol + rtol·max(|y_i|, |y_new_i|)) ≤ 1, next step
  * h·0.9·e^(−1/4) in [h/10, 5h], |h| within [1e-8, 8]·|h0| (below: DIVERGED).
+ * A trial step with a non-finite component is rejected with h/10.
  * max_steps counts accepted steps; rejected (optional) the rejected attempts.
  * h0 ≠ 0, rtol ≥ 0, atol > 0. Outputs as geon_trace_rays (no residual). */
 int  geon_trace_rays_adaptive(geon_ctx_t* ctx, const geon_metric_t* metric, const double* y0_soa,

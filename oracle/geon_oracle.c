@@ -410,7 +410,8 @@ void or_rk4_step(const geon_metric_t* m, const double y[8], double h, double out
  * as the next step). Cash–Karp RK4(5) (Cash & Karp, ACM TOMS 16 (1990)
  * 201-222) on or_rhs, the error test max_i |err_i|/(atol + rtol·max(|y_i|,
  * |y_new_i|)) ≤ 1, next step h·0.9·e^(-1/4) in [h/10, 5h] (5h below
- * e = 1.0498e-3), |h| in [1e-8, 8]·|h0|; every sum left to right in the
+ * e = 1.0498e-3), a trial step with a non-finite component rejected with
+ * h/10, |h| in [1e-8, 8]·|h0|; every sum left to right in the
  * tableau's order. No reference result exists for it; the tests pin it by
  * the tableau's orders (local errors O(h^6) and O(h^5) of the fifth- and
  * embedded fourth-order solutions), closed forms (straight lines, conserved
@@ -466,15 +467,16 @@ void or_trace_adaptive_rays(const geon_metric_t* m, const double* y0, size_t n, 
                 const double sc = atol + rtol * fmax(fabs(y[i]), fabs(yn[i]));
                 e = fmax(e, fabs(err[i]) / sc);
             }
-            if (e > 1.0) {
+            /* a non-finite trial step is rejected with the factor 0.1 (fmax
+             * above drops a NaN from e); DIVERGED only below h_min */
+            int fin = 1;
+            for (int i = 0; i < 8; ++i) fin = fin && isfinite(yn[i]) && isfinite(err[i]);
+            if (!fin || e > 1.0) {
                 ++rej;
-                h = h * fmax(0.9 / sqrt(sqrt(e)), 0.1);
+                h = h * (!fin ? 0.1 : fmax(0.9 / sqrt(sqrt(e)), 0.1));
                 if (!(fabs(h) >= h_min)) { c = 0; break; }
                 continue;
             }
-            int fin = 1;
-            for (int i = 0; i < 8; ++i) fin = fin && isfinite(yn[i]);
-            if (!fin) { c = 0; break; }
             memcpy(y, yn, sizeof y);
             ++k;
             h = h * (e > 1.0498e-3 ? 0.9 / sqrt(sqrt(e)) : 5.0);

@@ -123,3 +123,22 @@ def test_kernel_code_equals_oracle_bit_for_bit(hd, O, kind, kw, obs):
     assert np.array_equal(cls, ref["cls"]) and np.array_equal(steps, ref["steps"])
     assert np.array_equal(rej, ref["rejected"])
     assert np.array_equal(Y.view(np.uint64), ref["y"].view(np.uint64))
+
+
+def test_non_finite_trial_step_is_rejected(hd, O):
+    """A Schwarzschild ray falling in from r = 2.5M with h0 = 2.5: the first
+    Cash–Karp trial step reaches r ≤ 0 and comes out non-finite (its error
+    norm drops the NaN under fmax). The controller rejects it with h/10 and
+    retries instead of ending the ray as diverged: the ray is captured after
+    a few rejections, in the oracle and in the kernels' code built for the
+    host alike."""
+    m = O.metric(O.SCHWARZSCHILD, mass=1.0, horizon_pad=1.001, escape_radius=100.0)
+    f = 1.0 - 2.0 / 2.5
+    y0 = np.array([[0.0, 2.5, math.pi / 2, 0.0, 1.0 / f, -1.0, 0.0, 0.0]])
+    yn, err = O.ck_step(m, y0[0], 2.5)
+    assert not (np.isfinite(yn).all() and np.isfinite(err).all())
+    ref = O.trace_rays_adaptive(m, y0, 2.5, 1e-9, 1e-12, 1000)
+    assert ref["cls"][0] == O.CAPTURED and ref["rejected"][0] >= 1 and ref["steps"][0] > 0
+    cls, steps, rej, Y = _adaptive_host(hd, m, y0, 2.5, 1e-9, 1e-12, 1000)
+    assert cls[0] == ref["cls"][0] and steps[0] == ref["steps"][0] and rej[0] == ref["rejected"][0]
+    assert np.array_equal(Y.view(np.uint64), ref["y"].view(np.uint64))
