@@ -11,7 +11,10 @@
 // the kernel and slot it runs in; written out, it is fixed by the source.
 // After each step p^t is re-projected onto the null cone g(p,p) = 0
 // (future-directed root), which keeps FP32 drift out of the
-// energy-carrying component. Where Γ vanishes at every stage (Minkowski,
+// energy-carrying component — except in the Alcubierre wall, where the
+// reference's own step leaves the cone by its truncation error and the
+// re-projection would change the map that the FP64 trace follows (see
+// AlcubierreF::reproject). Where Γ vanishes at every stage (Minkowski,
 // Alcubierre beyond its wall) the step is the straight line y + h·p.
 // Rays whose outcome is sensitive to rounding
 // are handed to the FP64 path: a priori those whose impact parameter lies
@@ -285,18 +288,17 @@ struct AlcubierreF {
         dp[2] = 0.5f * (n[1] * __builtin_fmaf(Ac * p0, p0, b2));
         dp[3] = 0.5f * (n[2] * __builtin_fmaf(Ac * p0, p0, b2));
     }
-    GEON_D static void reproject(const FParams& P, float* y) {
-        float f, fp, n[3];
-        shape(P, y, f, fp, n);
-        const float vf = P.v * f;
-        const float a = __builtin_fmaf(vf, vf, -1.0f);   // g_tt
-        if (a < -1e-3f) {                         // outside any horizon of the wall
-            const float b = -vf * y[5];           // g_tx p^x
-            const float c = norm3sq(y[5], y[6], y[7]);
-            const float disc = __builtin_fmaf(b, b, -(a * c));
-            if (disc >= 0.0f) y[4] = (b + fsqrt(disc)) * frcp(-a);
-        }
-    }
+    // No null re-projection here. Inside the wall the reference's RK4 step
+    // drifts off the null cone by its own truncation error, and the FP64
+    // trace it is compared with keeps that drift; moving p^t back onto the
+    // cone every step changes the discrete map itself, and the change grows
+    // across the wall. On the CPU, an FP32 emulation of this path against
+    // the oracle (v = 0.9, 4000 rays through the wall region): with the
+    // re-projection 1496 escape directions off by more than 1e-3 rad (max
+    // 2.4 rad), without it 10 (max 4.0e-3), all after more than 256 wall
+    // steps. Outside the wall the momentum is constant and there is nothing
+    // to re-project.
+    GEON_D static void reproject(const FParams&, float*) {}
     GEON_D static int terminate(const FParams& P, const float* x) {
         const float r2 = norm3sq(x[1], x[2], x[3]);
         if (!__builtin_isfinite(r2)) return DIVERGED;
@@ -305,18 +307,19 @@ struct AlcubierreF {
     // The second tracker counts the steps that evaluated Γ (wall steps).
     // Rays that stay in the wall region long — those that cross the bubble's
     // interior, or are caught near its walls — have escape directions that
-    // amplify FP32 rounding (up to radians), and trapped ones change class
-    // between escaped and max-steps. A ray that took more than kWallSteps
-    // such steps, or is still inside the escape sphere at max_steps, is
-    // re-traced in FP64. Measured on the 94 Alcubierre frames of the
-    // 256-frame ragged batch (v ∈ [0.1, 0.9], R = 2, s = 1/8), FP32-mixed
-    // against FP64: escape-angle error max 2.0 rad with 1.5 M of 54 M pixels
-    // above 1e-3 rad at 1024; 0.52 rad and 1.2 M at 256; 1.9e-4 rad and none
-    // at 128 (10.7 % of the rays re-traced); no class mismatch from 128 to
-    // 1024 once max-steps rays are re-traced.
+    // amplify FP32 rounding, and trapped ones change class between escaped
+    // and max-steps. A ray that took more than kWallSteps such steps, or is
+    // still inside the escape sphere at max_steps, is re-traced in FP64.
+    // Measured on the 94 Alcubierre frames of the 256-frame ragged batch
+    // (v ∈ [0.1, 0.9], R = 2, s = 1/8, 53.8 M pixels), FP32-mixed against
+    // FP64, no class mismatch at any threshold; escape directions above
+    // 1e-3 rad: 564 pixels (max 1.7e-2 rad) at 192, 872 (max 1.7e-2) at 256
+    // with 0.7 % of the rays re-traced, 2188 (max 5.5e-2) at 384. With the
+    // re-projection the same frames needed 128 (10.7 % re-traced; 1.2 M
+    // pixels above 1e-3 rad, max 0.52 rad, at 256).
     static constexpr bool counts_wall = true;
 #ifndef GEON_FP32_WALL_STEPS
-#define GEON_FP32_WALL_STEPS 128
+#define GEON_FP32_WALL_STEPS 256
 #endif
     enum { kWallSteps = GEON_FP32_WALL_STEPS };
     GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float, float wall) {
