@@ -355,14 +355,26 @@ def _acc_report(a):
             "pixels": a["pixels"]}
 
 
+_TUNED = set()
+
+
+def tune_kinds(geon, ctx):
+    """Autotune the scheduler's launch for every metric kind of the batch and
+    serving workloads in both precisions, once per context."""
+    if id(ctx) in _TUNED:
+        return
+    for prec in (geon.PREC_FP64, geon.PREC_MIXED):
+        for kind in (geon.SCHWARZSCHILD, geon.ALCUBIERRE, geon.MORRIS_THORNE):
+            geon.autotune(kind, prec, ctx=ctx)
+    _TUNED.add(id(ctx))
+
+
 def ragged_batch(geon, ctx, torch, stream, ev):
     from geon_amd import workloads
     res = {}
     outs = {}
+    tune_kinds(geon, ctx)
     for label, prec in (("fp64", geon.PREC_FP64), ("mixed", geon.PREC_MIXED)):
-        geon.autotune(geon.SCHWARZSCHILD, prec, ctx=ctx)
-        geon.autotune(geon.ALCUBIERRE, prec, ctx=ctx)
-        geon.autotune(geon.MORRIS_THORNE, prec, ctx=ctx)
         frames = workloads.ragged_frames(precision=prec)
         bt = geon.Batch(ctx)
         o = []
@@ -413,6 +425,7 @@ def serving(geon, ctx, torch, stream, outs, ev, seconds):
                        "FP32-mixed), 20% bisections 64 rays × 40 halvings (M=1, r_obs∈[10,50], "
                        "photon-sphere decisions); seed 7; 8 GiB admission budget, 2048-step quanta",
            "rates": {}}
+    tune_kinds(geon, ctx)
     for rate in (50, 100, 200, 400):
         reqs = workloads.serving_stream(rate, int(rate * seconds))
         rng = np.random.default_rng(rate)

@@ -12,9 +12,17 @@
 //      quantum, so rays of newly admitted requests start next to rays
 //      already in flight;
 //   3. polls the events; when a pipeline's quantum ends it harvests that
-//      kind's finished jobs: renders get their per-pixel reduction,
-//      bisection requests walk their k-section trees and enqueue the next
-//      round of probes.
+//      kind's finished jobs: renders get their per-pixel reduction (enqueued
+//      on the pipeline with the read-back of their statistics; the request
+//      completes when that work's event has passed), bisection requests walk
+//      their k-section trees and start the next round of probes.
+// The loop never waits for the device: job descriptors are staged in pinned
+// memory and copied on the pipeline's stream ahead of its next quantum, and
+// the probe rays of bisections keep their initial states and outcomes in
+// pinned host memory that the kernels read and write directly. A blocking
+// copy or stream synchronisation here would wait for the other pipelines'
+// persistent kernels to free a slot for the copy, and every pipeline would
+// stall with it.
 // The FP64 hand-back list of a kind is drained while FP32 work no longer
 // fills the GPU or once it holds a quarter of the FP64 slots, so FP64
 // quanta run at useful occupancy under load.
@@ -23,6 +31,7 @@
 #include <math.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -41,6 +50,7 @@ namespace {
 const double kHalfPi = 1.5707963267948966;
 const int kJobWords = 2 + (int)GEON_DSTATS_WORDS;  // per job: done, fp64, geon_dstats
 const int kNoRoom = 1;     // internal: the arena cannot hold the request yet
+const size_t kProbePool = 64ull << 20;   // pinned bytes for bisection probe rays
 
 double now_ms() {
     using namespace std::chrono;
@@ -93,7 +103,10 @@ struct Arena {
 struct Queue {
     geon_djob* d_jobs = nullptr;
     unsigned long long* d_base = nullptr;
+    geon_djob* h_jobs = nullptr;           // pinned copies, written once per job
+    unsigned long long* h_base = nullptr;
     uint32_t n_jobs = 0;
+    uint32_t n_up = 0;                     // jobs already copied to the device
     uint64_t end = 0;
     geon_sched_cfg_t cfg;
     uint32_t lanes = 0, n_slots = 0;
@@ -103,6 +116,8 @@ struct Queue {
     float* slot_aux = nullptr;
     int outstanding = 0;       // jobs not yet complete
 };
+
+struct Request;
 
 // A metric kind's quantum pipeline: its stream, the event that ends the
 // last enqueued quantum and pinned read-back buffers.
@@ -115,15 +130,20 @@ struct Pipe {
     int lo = 0, hi = -1;
     bool ran32 = false, ranfb = false;    // queues run by the quantum in flight
     unsigned long long parked32 = 0, parkedfb = 0;  // rays parked by their last quanta
+    // renders whose reduction and statistics read-back are enqueued, with
+    // the event that ends that work
+    std::deque<std::pair<hipEvent_t, std::vector<std::shared_ptr<Request>>>> fin;
+    std::vector<hipEvent_t> spare;        // events for fin
 };
 
 struct Request {
     uint64_t ticket = 0;
     int kind = GEON_REQ_RENDER;
-    int state = 0;             // 0 queued, 1 running, 2 done
+    int state = 0;             // 0 queued, 1 running, 2 done, 3 reduction enqueued
     double t_submit = 0, t_admit = 0, t_done = 0;
     size_t footprint = 0;
-    std::vector<std::pair<size_t, size_t>> ranges;  // arena allocations
+    std::vector<std::pair<size_t, size_t>> ranges;   // HBM arena allocations
+    std::vector<std::pair<size_t, size_t>> hranges;  // pinned pool allocations
     // render
     geon_metric_t metric;
     geon_camera_t cam;
@@ -138,7 +158,7 @@ struct Request {
     double step = 0.01;
     uint32_t max_steps = 0, iters = 0, done_iters = 0, levels = 0;
     bool full_trace = false;
-    double* d_y0 = nullptr;
+    double* d_y0 = nullptr;    // pinned host memory, read and written by the kernels
     int32_t* d_cls = nullptr;
     uint32_t* d_steps = nullptr;
     std::vector<uint32_t> round_steps;
@@ -157,6 +177,9 @@ struct geon_server {
     geon_serve_cfg_t cfg;
     Arena arena;
     void* arena_mem = nullptr;
+    Arena hpool;                         // pinned, device-visible: bisection probes
+    void* hpool_mem = nullptr;
+    unsigned long long* h_fin = nullptr; // pinned: per job, its words after the reduction
     Queue q[4][3];                       // [kind][0 FP64 | 1 FP32 | 2 FP64 probes, one per lane]
     Queue fb[4];                         // FP64 pass over FP32 hand-backs
     unsigned long long* d_list[4] = {};  // fallback ray ids
@@ -197,9 +220,20 @@ int alloc_req(geon_server* s, Request& r, size_t bytes, void** p) {
     return GEON_OK;
 }
 
+// the same from the pinned probe pool (device-visible host memory)
+int alloc_host(geon_server* s, Request& r, size_t bytes, void** p) {
+    size_t off;
+    if (!s->hpool.alloc(bytes, &off)) return kNoRoom;
+    r.hranges.push_back({off, bytes});
+    *p = s->hpool.base + off;
+    return GEON_OK;
+}
+
 void free_req(geon_server* s, Request& r) {
     for (auto& rg : r.ranges) s->arena.release(rg.first, rg.second);
     r.ranges.clear();
+    for (auto& rg : r.hranges) s->hpool.release(rg.first, rg.second);
+    r.hranges.clear();
 }
 
 int init_queue(geon_server* s, Queue& Q, int kind, int prec, uint32_t cap_jobs, bool one_per_lane) {
@@ -212,6 +246,10 @@ int init_queue(geon_server* s, Queue& Q, int kind, int prec, uint32_t cap_jobs, 
     const size_t tsz = prec ? 4 : 8;
     GEON_HIP(hipMalloc((void**)&Q.d_jobs, (size_t)cap_jobs * sizeof(geon_djob)), "jobs");
     GEON_HIP(hipMalloc((void**)&Q.d_base, (size_t)cap_jobs * 8), "bases");
+    GEON_HIP(hipHostMalloc((void**)&Q.h_jobs, (size_t)cap_jobs * sizeof(geon_djob),
+                           hipHostMallocDefault), "jobs (pinned)");
+    GEON_HIP(hipHostMalloc((void**)&Q.h_base, (size_t)cap_jobs * 8, hipHostMallocDefault),
+             "bases (pinned)");
     GEON_HIP(hipMalloc(&Q.slot_y, (size_t)Q.n_slots * 8 * tsz), "slots");
     GEON_HIP(hipMalloc((void**)&Q.slot_g, (size_t)Q.n_slots * 8), "slots");
     GEON_HIP(hipMalloc((void**)&Q.slot_n, (size_t)Q.n_slots * 4), "slots");
@@ -226,6 +264,8 @@ int init_queue(geon_server* s, Queue& Q, int kind, int prec, uint32_t cap_jobs, 
 void free_queue(Queue& Q) {
     if (Q.d_jobs) hipFree(Q.d_jobs);
     if (Q.d_base) hipFree(Q.d_base);
+    if (Q.h_jobs) hipHostFree(Q.h_jobs);
+    if (Q.h_base) hipHostFree(Q.h_base);
     if (Q.slot_y) hipFree(Q.slot_y);
     if (Q.slot_g) hipFree(Q.slot_g);
     if (Q.slot_n) hipFree(Q.slot_n);
@@ -242,9 +282,10 @@ int add_job(geon_server* s, int kind, int prec, geon_djob& dj, int* job) {
     dj.fp64 = w + 1;
     dj.stats = (geon_dstats*)(w + 2);
     dj.ray_base = Q.end;
-    // fresh counters (zeroed at creation, never reused); upload descriptor
-    GEON_HIP(hipMemcpyHtoD((hipDeviceptr_t)(Q.d_jobs + Q.n_jobs), &dj, sizeof dj), "job upload");
-    GEON_HIP(hipMemcpyHtoD((hipDeviceptr_t)(Q.d_base + Q.n_jobs), &dj.ray_base, 8), "base upload");
+    // fresh counters (zeroed at creation, never reused); the descriptor is
+    // staged here and copied on the kind's stream before its next quantum
+    Q.h_jobs[Q.n_jobs] = dj;
+    Q.h_base[Q.n_jobs] = dj.ray_base;
     s->last_qpos = Q.n_jobs;
     Q.n_jobs++;
     Q.end += dj.n_rays;
@@ -299,7 +340,7 @@ int start_round(geon_server* s, Request& r) {
             const size_t idx = k * nodes + i;
             for (int c = 0; c < 8; ++c) y0[(size_t)c * nr + idx] = y[c];
         }
-    GEON_HIP(hipMemcpyHtoD((hipDeviceptr_t)r.d_y0, y0.data(), y0.size() * 8), "probe upload");
+    memcpy(r.d_y0, y0.data(), y0.size() * 8);   // pinned: no kernel reads it until the next quantum
     // problems may differ in mass: one job per distinct mass keeps the
     // metric constants per job; here every problem of a request shares the
     // job when masses are equal, else one job per problem.
@@ -361,11 +402,11 @@ int admit(geon_server* s, Request& r) {
     }
     const size_t n = r.mass.size();
     void* p;
-    GEON_TRY(alloc_req(s, r, n * 63 * 8 * 8, &p));
+    GEON_TRY(alloc_host(s, r, n * 63 * 8 * 8, &p));
     r.d_y0 = (double*)p;
-    GEON_TRY(alloc_req(s, r, n * 63 * 4, &p));
+    GEON_TRY(alloc_host(s, r, n * 63 * 4, &p));
     r.d_cls = (int32_t*)p;
-    GEON_TRY(alloc_req(s, r, n * 63 * 4, &p));
+    GEON_TRY(alloc_host(s, r, n * 63 * 4, &p));
     r.d_steps = (uint32_t*)p;
     r.lo.assign(n, 0.0);
     r.hi.assign(n, kHalfPi - 1e-3);
@@ -373,6 +414,8 @@ int admit(geon_server* s, Request& r) {
     return start_round(s, r);
 }
 
+// Bytes a request holds while it runs: of the HBM arena for a render, of the
+// pinned probe pool for a bisection.
 size_t footprint(const Request& r) {
     if (r.kind == GEON_REQ_RENDER) {
         const uint64_t n_pix = (uint64_t)r.cam.width * r.cam.height;
@@ -421,19 +464,27 @@ int launch_quantum(geon_server* s, Queue& Q, int kind, int f32, unsigned long lo
     return GEON_OK;
 }
 
-int finish_render(geon_server* s, Request& r, hipStream_t st) {
+// A render whose rays are all done: enqueue its per-pixel reduction and the
+// read-back of its job words (statistics) on the pipeline's stream.
+int start_finish_render(geon_server* s, Request& r, hipStream_t st) {
     geon_ctx* c = s->ctx;
     Queue& Q = s->q[r.mk][r.mq];
-    // job index within its queue: jobs are appended in order, find it by base
+    // job index within its queue: jobs are appended in order
     const geon_djob* dptr = Q.d_jobs + r.qpos;
     void* fargs[] = {&dptr};
     const uint64_t n_pix = (uint64_t)r.cam.width * r.cam.height;
     GEON_HIP(hipModuleLaunchKernel(c->fn_job_finalize, (uint32_t)((n_pix + 255) / 256), 1, 1, 256, 1,
                                    1, 0, st, fargs, nullptr), "finalize");
-    unsigned long long w[kJobWords];
-    GEON_HIP(hipMemcpyDtoHAsync(w, (hipDeviceptr_t)(s->d_job + (size_t)r.job * kJobWords), sizeof w,
-                                st), "stats");
-    GEON_HIP(hipStreamSynchronize(st), "sync");
+    GEON_HIP(hipMemcpyDtoHAsync(s->h_fin + (size_t)r.job * kJobWords,
+                                (hipDeviceptr_t)(s->d_job + (size_t)r.job * kJobWords),
+                                kJobWords * 8, st), "stats");
+    return GEON_OK;
+}
+
+// The same render once the event after that work has passed.
+void finish_render(geon_server* s, Request& r) {
+    const uint64_t n_pix = (uint64_t)r.cam.width * r.cam.height;
+    const unsigned long long* w = s->h_fin + (size_t)r.job * kJobWords;
     geon_dstats ds;
     memcpy(&ds, w + 2, sizeof ds);
     geon_render_stats_t& S = r.result.stats;
@@ -449,18 +500,15 @@ int finish_render(geon_server* s, Request& r, hipStream_t st) {
     S.ray_steps = ds.ray_steps;
     S.fp64_rays = r.mq ? w[1] : r.n_rays;
     r.ray_steps = ds.ray_steps;
-    return GEON_OK;
 }
 
 // After a k-section round: walk each problem's tree with the round's classes.
 int finish_round(geon_server* s, Request& r, bool* more) {
     const uint32_t nodes = (1u << r.levels) - 1u;
     const size_t n = r.mass.size();
-    r.round_cls.resize(n * nodes);
-    r.round_steps.resize(n * nodes);
-    GEON_HIP(hipMemcpyDtoH(r.round_cls.data(), (hipDeviceptr_t)r.d_cls, r.round_cls.size() * 4), "cls");
-    GEON_HIP(hipMemcpyDtoH(r.round_steps.data(), (hipDeviceptr_t)r.d_steps, r.round_steps.size() * 4),
-             "steps");
+    // pinned memory written by the quantum whose event has passed
+    r.round_cls.assign(r.d_cls, r.d_cls + n * nodes);
+    r.round_steps.assign(r.d_steps, r.d_steps + n * nodes);
     for (uint32_t v : r.round_steps) r.ray_steps += v;
     for (size_t k = 0; k < n; ++k) {
         uint32_t i = 1;
@@ -492,6 +540,19 @@ int finish_round(geon_server* s, Request& r, bool* more) {
     return GEON_OK;
 }
 
+// Copy the descriptors of jobs added since the last upload (pinned source,
+// written once) on stream st.
+int upload_jobs(Queue& Q, hipStream_t st) {
+    if (Q.n_up >= Q.n_jobs) return GEON_OK;
+    const size_t n = Q.n_jobs - Q.n_up;
+    GEON_HIP(hipMemcpyHtoDAsync((hipDeviceptr_t)(Q.d_jobs + Q.n_up), Q.h_jobs + Q.n_up,
+                                n * sizeof(geon_djob), st), "job upload");
+    GEON_HIP(hipMemcpyHtoDAsync((hipDeviceptr_t)(Q.d_base + Q.n_up), Q.h_base + Q.n_up, n * 8, st),
+             "base upload");
+    Q.n_up = Q.n_jobs;
+    return GEON_OK;
+}
+
 // Enqueue one quantum of every active queue of kind k, then the read-back
 // of the kind's counters and of its running jobs' completion words.
 int launch_kind(geon_server* s, int k) {
@@ -502,6 +563,10 @@ int launch_kind(geon_server* s, int k) {
     Queue& FB = s->fb[k];
     unsigned long long* ck = s->d_ctr + (size_t)k * K_WORDS;
     const hipStream_t st = P.stream;
+    // job descriptors added since the last quantum, ahead of the kernels
+    GEON_TRY(upload_jobs(Q32, st));
+    GEON_TRY(upload_jobs(Q64, st));
+    GEON_TRY(upload_jobs(s->q[k][2], st));
     unsigned long long* park = ck + K_PARK32;
     uint64_t n_park = 4;
     void* za[] = {&park, &n_park};
@@ -531,7 +596,7 @@ int launch_kind(geon_server* s, int k) {
     GEON_HIP(hipMemcpyDtoHAsync(P.h_ctr, (hipDeviceptr_t)ck, K_WORDS * 8, st), "counters");
     int lo = 1 << 30, hi = -1;
     for (auto& r : s->running) {
-        if (r->mk != k) continue;
+        if (r->mk != k || r->state == 3) continue;
         lo = r->job < lo ? r->job : lo;
         hi = r->job > hi ? r->job : hi;
     }
@@ -546,41 +611,94 @@ int launch_kind(geon_server* s, int k) {
     return GEON_OK;
 }
 
-// After kind k's quantum: finish the jobs whose rays are all done.
+// Mark a request done and release what it no longer holds.
+void complete(geon_server* s, Request& r, int st) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    r.result.status = st;
+    r.result.ray_steps = r.ray_steps;
+    r.t_done = now_ms();
+    r.result.latency_ms = r.t_done - r.t_submit;
+    r.result.queue_ms = r.t_admit - r.t_submit;
+    r.state = 2;
+    if (!r.keep || r.kind == GEON_REQ_BISECT) free_req(s, r);
+    s->n_done++;
+    s->steps_done += r.ray_steps;
+    s->done_cv.notify_all();
+}
+
+// After kind k's quantum: finish the jobs whose rays are all done. Renders
+// get their reduction enqueued and complete in reap_kind; bisections walk
+// their trees now (their probe outcomes are in pinned memory).
 void harvest_kind(geon_server* s, int k) {
     Pipe& P = s->pipe[k];
-    std::vector<std::shared_ptr<Request>> still;
+    std::vector<std::shared_ptr<Request>> still, fin;
     for (auto& r : s->running) {
-        if (r->mk != k || r->job < P.lo || r->job > P.hi ||
+        if (r->state == 3 || r->mk != k || r->job < P.lo || r->job > P.hi ||
             P.h_job[(size_t)(r->job - P.lo) * kJobWords] < r->n_rays) {
             still.push_back(r);
             continue;
         }
         s->q[r->mk][r->mq].outstanding--;
-        int st = GEON_OK;
-        bool more = false;
         if (r->kind == GEON_REQ_RENDER) {
-            st = finish_render(s, *r, P.stream);
-        } else {
-            st = finish_round(s, *r, &more);
+            const int st = start_finish_render(s, *r, P.stream);
+            if (st != GEON_OK) {
+                complete(s, *r, st);
+                continue;
+            }
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                r->state = 3;
+            }
+            fin.push_back(r);
+            still.push_back(r);
+            continue;
         }
+        bool more = false;
+        const int st = finish_round(s, *r, &more);
         if (more && st == GEON_OK) {
             still.push_back(r);
             continue;
         }
-        std::lock_guard<std::mutex> lk(s->mu);
-        r->result.status = st;
-        r->result.ray_steps = r->ray_steps;
-        r->t_done = now_ms();
-        r->result.latency_ms = r->t_done - r->t_submit;
-        r->result.queue_ms = r->t_admit - r->t_submit;
-        r->state = 2;
-        if (!r->keep || r->kind == GEON_REQ_BISECT) free_req(s, *r);
-        s->n_done++;
-        s->steps_done += r->ray_steps;
-        s->done_cv.notify_all();
+        complete(s, *r, st);
     }
     s->running.swap(still);
+    if (fin.empty()) return;
+    hipEvent_t ev = nullptr;
+    if (!P.spare.empty()) {
+        ev = P.spare.back();
+        P.spare.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        ev = nullptr;
+    }
+    if (!ev || hipEventRecord(ev, P.stream) != hipSuccess) {
+        if (ev) P.spare.push_back(ev);
+        for (auto& r : fin) complete(s, *r, fail(GEON_E_HIP, "event"));
+        s->running.erase(std::remove_if(s->running.begin(), s->running.end(),
+                                        [](const std::shared_ptr<Request>& r) { return r->state == 2; }),
+                         s->running.end());
+        return;
+    }
+    P.fin.emplace_back(ev, std::move(fin));
+}
+
+// Complete the renders of kind k whose reduction has finished.
+bool reap_kind(geon_server* s, int k) {
+    Pipe& P = s->pipe[k];
+    bool any = false;
+    while (!P.fin.empty() && hipEventQuery(P.fin.front().first) == hipSuccess) {
+        for (auto& r : P.fin.front().second) {
+            finish_render(s, *r);
+            complete(s, *r, GEON_OK);
+        }
+        P.spare.push_back(P.fin.front().first);
+        P.fin.pop_front();
+        any = true;
+    }
+    if (any)
+        s->running.erase(std::remove_if(s->running.begin(), s->running.end(),
+                                        [](const std::shared_ptr<Request>& r) { return r->state == 2; }),
+                         s->running.end());
+    return any;
 }
 
 void serve_loop(geon_server* s) {
@@ -589,7 +707,7 @@ void serve_loop(geon_server* s) {
     double busy_since = -1.0;
     for (;;) {
         bool any_busy = false;
-        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy;
+        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy || !s->pipe[k].fin.empty();
         // ---- admission ----
         {
             std::unique_lock<std::mutex> lk(s->mu);
@@ -600,7 +718,7 @@ void serve_loop(geon_server* s) {
             }
             while (!s->pending.empty()) {
                 auto r = s->pending.front();
-                if (r->footprint > s->arena.size) {
+                if (r->footprint > (r->kind == GEON_REQ_RENDER ? s->arena.size : s->hpool.size)) {
                     r->result.status = GEON_E_ADMISSION;
                     r->state = 2;
                     r->t_done = now_ms();
@@ -639,7 +757,7 @@ void serve_loop(geon_server* s) {
             }
         }
         any_busy = false;
-        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy;
+        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy || !s->pipe[k].fin.empty();
         if (any_busy && busy_since < 0.0) busy_since = now_ms();
         // ---- harvest pipelines whose quantum ended ----
         bool progressed = false;
@@ -652,8 +770,9 @@ void serve_loop(geon_server* s) {
             if (P.ranfb) P.parkedfb = P.h_ctr[K_PARKFB];
             harvest_kind(s, k);
         }
+        for (int k = 0; k < 4; ++k) progressed = reap_kind(s, k) || progressed;
         any_busy = false;
-        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy;
+        for (int k = 0; k < 4; ++k) any_busy = any_busy || s->pipe[k].busy || !s->pipe[k].fin.empty();
         if (!any_busy && busy_since >= 0.0) {
             s->busy_ms += now_ms() - busy_since;
             busy_since = -1.0;
@@ -697,6 +816,13 @@ int geon_serve_create(geon_ctx_t* ctx, const geon_serve_cfg_t* cfg, geon_server_
     if (hipMalloc(&s->arena_mem, s->cfg.hbm_budget) != hipSuccess)
         return bail(fail(GEON_E_NOMEM, "arena of %llu bytes", (unsigned long long)s->cfg.hbm_budget));
     s->arena.init((char*)s->arena_mem, s->cfg.hbm_budget);
+    // probe pool: coherent pinned memory the scheduler reads and writes
+    // directly (4.6 KB per bisection problem: ~230 requests of 64 problems)
+    if (hipHostMalloc(&s->hpool_mem, kProbePool, hipHostMallocCoherent) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_fin, (size_t)s->cfg.max_jobs * kJobWords * 8,
+                      hipHostMallocDefault) != hipSuccess)
+        return bail(fail(GEON_E_NOMEM, "pinned pools"));
+    s->hpool.init((char*)s->hpool_mem, kProbePool);
     for (int k = 0; k < 4 && st == GEON_OK; ++k) {
         for (int q = 0; q < 2 && st == GEON_OK; ++q)
             st = init_queue(s, s->q[k][q], k, q, s->cfg.max_jobs, false);
@@ -745,10 +871,14 @@ void geon_serve_destroy(geon_server_t* s) {
     if (s->d_ctr) hipFree(s->d_ctr);
     if (s->d_job) hipFree(s->d_job);
     if (s->arena_mem) hipFree(s->arena_mem);
+    if (s->hpool_mem) hipHostFree(s->hpool_mem);
+    if (s->h_fin) hipHostFree(s->h_fin);
     for (int k = 0; k < 4; ++k) {
         Pipe& P = s->pipe[k];
         if (P.stream) hipStreamDestroy(P.stream);
         if (P.done) hipEventDestroy(P.done);
+        for (auto& f : P.fin) P.spare.push_back(f.first);
+        for (hipEvent_t e : P.spare) hipEventDestroy(e);
         if (P.h_ctr) hipHostFree(P.h_ctr);
         if (P.h_job) hipHostFree(P.h_job);
     }
