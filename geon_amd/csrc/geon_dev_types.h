@@ -25,7 +25,7 @@ struct geon_dmetric {
     double  fd_inv_2h;   // 1.0 / (2.0 * h)                   (curvature.rs:34)
     double  fd_sin_h, fd_cos_h;   // sin h, cos h: θ ± h rotations (Morris–Thorne)
     double  inv_denom;   // 1.0 / denom (Alcubierre)
-    int32_t mk;          // 1: quotients by 2h and denom may take Markstein's form (see make_dmetric)
+    int32_t mk;          // 1: quotients may take the shorter sequences (Alcubierre: by 2h and denom; Schwarzschild: M in range; see make_dmetric)
     int32_t pad_;
 };
 

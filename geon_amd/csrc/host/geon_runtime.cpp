@@ -155,6 +155,10 @@ int make_dmetric(const geon_metric_t* m, geon_dmetric* d) {
         d->mk = (d->s > 0.0 && d->denom >= 1e-3 && d->denom <= 2.0 && fabs(d->v) <= 1e6 &&
                  d->fd_two_h >= 1e-150 && d->fd_two_h <= 1e150) ? 1 : 0;
     }
+    // Schwarzschild's quotients take the shorter sequences where M, r and
+    // f = 1 − 2M/r lie within 2^±100 (the kernels test r and f)
+    if (m->kind == GEON_METRIC_SCHWARZSCHILD)
+        d->mk = (d->M >= 0x1p-100 && d->M <= 0x1p100) ? 1 : 0;
     if (m->kind == GEON_METRIC_MORRIS_THORNE) {
         d->b0 = m->p[0];
         d->fd_sin_h = sin(m->fd_step);

@@ -76,6 +76,26 @@ GEON_D double rcp_rn(double x) {
     return 1.0 / x;
 #endif
 }
+// RN(a/b) by the compiler's IEEE division sequence for a/b (hardware
+// reciprocal, two Newton steps, the quotient and its FMA remainder
+// correction) without the operand scaling and the special-case fix-up,
+// which leave the operands and the quotient unchanged while a, b and a/b
+// are normal numbers within 2^±900 of 1: the same bits in 8 instructions
+// instead of 11. Callers guard the range.
+GEON_D double div_ns(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double y = __builtin_amdgcn_rcp(b);
+    double e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-b, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    const double q = a * y;
+    const double r = __builtin_fma(-b, q, a);
+    return __builtin_fma(r, y, q);
+#else
+    return a / b;
+#endif
+}
 #endif
 
 // A constant pinned to an SGPR pair where it is used. Loop-invariant FP64
@@ -130,6 +150,101 @@ GEON_D void sincos_cw(double x, double* s, double* c) {
     const double cv = (q & 1) ? sr : cr;
     *s = (q & 2) ? -sv : sv;
     *c = ((q + 1) & 2) ? -cv : cv;
+}
+
+// sin x and cos x together to within about 0.75 ulp, for |x| < 2^20 (else
+// the library sincos), each also as a two-word value s + sl, c + cl within
+// about 0.2 ulp of s. The same Cody–Waite reduction and Taylor polynomials
+// as sincos_cw, with corrections that make the result nearly correctly
+// rounded: the reduced argument is kept as the exact two-word sum r + rl
+// (the FMA yields the rounding error of r − k·lo(π/2)), so sin(r + rl) = r +
+// (r³·S(r²) + rl·(1 − r²/2)) and cos(r + rl) = w + ((1 − w) − r²/2 − rl·r +
+// r⁴·C(r²)) with w = 1 − r²/2 rounded; the rounding error of w is recovered
+// exactly and that of r² by an FMA, so only the final sum rounds at the
+// scale of the result, and that sum's rounding error is the low word (exact:
+// the leading term dominates the correction). Against glibc's sin and cos
+// the rounded values differ in the last bit on about 1.5 % of arguments in
+// [0, π] (sincos_cw: 20–25 %; on gfx950 the library's sincos is further
+// from glibc's than this).
+GEON_D void sincos_acc_dw(double x, double* s, double* sl, double* c, double* cl) {
+    if (!(fabs(x) < 0x1p20)) {
+        sincos(x, s, c);
+        *sl = 0.0;
+        *cl = 0.0;
+        return;
+    }
+    const double k = rint(x * 0x1.45f306dc9c883p-1);          // x·(2/π)
+    const double r1 = __builtin_fma(-k, 0x1.921fb54442d18p+0, x);   // x − k·hi(π/2), exact
+    const double r = __builtin_fma(-k, 0x1.1a62633145c07p-54, r1);  // − k·lo(π/2)
+    const double rl = __builtin_fma(-k, 0x1.1a62633145c07p-54, r1 - r);   // its rounding error
+    const double z = r * r;
+    const double zl = __builtin_fma(r, r, -z);                 // r² = z + zl exactly
+    const double hz = 0.5 * z;
+    double ps = 0x1.952c77030ad4ap-49;                         //  1/17!
+    ps = __builtin_fma(ps, z, sconst(-0x1.ae7f3e733b81fp-41));      // −1/15!
+    ps = __builtin_fma(ps, z, sconst(0x1.6124613a86d09p-33));       //  1/13!
+    ps = __builtin_fma(ps, z, sconst(-0x1.ae64567f544e4p-26));      // −1/11!
+    ps = __builtin_fma(ps, z, sconst(0x1.71de3a556c734p-19));       //  1/9!
+    ps = __builtin_fma(ps, z, sconst(-0x1.a01a01a01a01ap-13));      // −1/7!
+    ps = __builtin_fma(ps, z, sconst(0x1.1111111111111p-7));        //  1/5!
+    ps = __builtin_fma(ps, z, sconst(-0x1.5555555555555p-3));       // −1/3!
+    const double ts = __builtin_fma(z * r, ps, __builtin_fma(-hz, rl, rl));
+    const double sr = r + ts;
+    const double srl = (r - sr) + ts;
+    double pc = -0x1.6827863b97d97p-53;                        // −1/18!
+    pc = __builtin_fma(pc, z, sconst(0x1.ae7f3e733b81fp-45));       //  1/16!
+    pc = __builtin_fma(pc, z, sconst(-0x1.93974a8c07c9dp-37));      // −1/14!
+    pc = __builtin_fma(pc, z, sconst(0x1.1eed8eff8d898p-29));       //  1/12!
+    pc = __builtin_fma(pc, z, sconst(-0x1.27e4fb7789f5cp-22));      // −1/10!
+    pc = __builtin_fma(pc, z, sconst(0x1.a01a01a01a01ap-16));       //  1/8!
+    pc = __builtin_fma(pc, z, sconst(-0x1.6c16c16c16c17p-10));      // −1/6!
+    pc = __builtin_fma(pc, z, sconst(0x1.5555555555555p-5));        //  1/4!
+    const double w = 1.0 - hz;                                 // hz ≤ 0.31: exact remainder
+    const double e = __builtin_fma(-0.5, zl, (1.0 - w) - hz);
+    const double tc = __builtin_fma(z * z, pc, __builtin_fma(-r, rl, e));
+    const double cr = w + tc;
+    const double crl = (w - cr) + tc;
+    const int q = (int)k & 3;
+    const bool sw = q & 1, ns = q & 2, nc = (q + 1) & 2;
+    const double sv = sw ? cr : sr, svl = sw ? crl : srl;
+    const double cv = sw ? sr : cr, cvl = sw ? srl : crl;
+    *s = ns ? -sv : sv;
+    *sl = ns ? -svl : svl;
+    *c = nc ? -cv : cv;
+    *cl = nc ? -cvl : cvl;
+}
+GEON_D void sincos_acc(double x, double* s, double* c) {
+    double sl, cl;
+    sincos_acc_dw(x, s, &sl, c, &cl);
+}
+
+// sin and cos of θ = θ0 + d from the two-word s0 + s0l = sin θ0 and c0 +
+// c0l = cos θ0 (sincos_acc_dw), for |d| ≤ 1/16, by rotation as sincos_rot:
+// sin θ = s0 + (s0l + s0·(cos d − 1) + (c0 + c0l)·sin d), cos θ = c0 + (c0l +
+// c0·(cos d − 1) − (s0 + s0l)·sin d). The correction is at most about |d|
+// times the larger of the two values, so its own rounding errors stay well
+// below an ulp of that value: within about 0.6 ulp of max(|sin θ|, |sin θ0|)
+// (the same for cos), i.e. about as close to the correctly rounded value as
+// sincos_acc's except where θ passes within |d| of a zero of the function,
+// where the error is that absolute amount. Larger |d| take sincos_acc.
+GEON_D void sincos_rot_dw(double th, double s0, double s0l, double c0, double c0l, double d,
+                          double* s, double* c) {
+    if (!(fabs(d) <= 0.0625)) {
+        sincos_acc(th, s, c);
+        return;
+    }
+    const double d2 = d * d;
+    double ps = __builtin_fma(0x1.71de3a556c734p-19, d2, sconst(-0x1.a01a01a01a01ap-13));  //  1/9!, −1/7!
+    ps = __builtin_fma(ps, d2, sconst(0x1.1111111111111p-7));                             //  1/5!
+    ps = __builtin_fma(ps, d2, sconst(-0x1.5555555555555p-3));                            // −1/3!
+    const double sd = __builtin_fma(d * d2, ps, d);
+    double pc = __builtin_fma(-0x1.27e4fb7789f5cp-22, d2, sconst(0x1.a01a01a01a01ap-16)); // −1/10!, 1/8!
+    pc = __builtin_fma(pc, d2, sconst(-0x1.6c16c16c16c17p-10));                           // −1/6!
+    pc = __builtin_fma(pc, d2, sconst(0x1.5555555555555p-5));                             //  1/4!
+    pc = __builtin_fma(pc, d2, -0.5);                                                     // −1/2!
+    const double cm = d2 * pc;
+    *s = s0 + __builtin_fma(s0, cm, __builtin_fma(c0, sd, __builtin_fma(c0l, sd, s0l)));
+    *c = c0 + __builtin_fma(c0, cm, __builtin_fma(-s0, sd, __builtin_fma(-s0l, sd, c0l)));
 }
 
 // tanh x in about a quarter of the instructions of the general library
@@ -241,6 +356,31 @@ GEON_D void stage_sincos(double th, double th0, double s0, double c0, double* s,
     sincos_rot(th, s0, c0, th - th0, s, c);
 #else
     (void)th0; (void)s0; (void)c0;
+    sincos(th, s, c);
+#endif
+}
+// Two-word sin θ and cos θ at a step's θ (sincos_acc_dw) and the rounded
+// values at its stages by rotation from them (sincos_rot_dw): for metrics
+// whose analytic Γ takes sin θ and cos θ at every stage and whose rays near
+// a photon sphere amplify last-bit differences orbit by orbit
+// (Schwarzschild). The host build evaluates the C library's at every stage,
+// as theta_sincos, unless GEON_HOST_DEVICE_MATH asks for the GPU's routines
+// (a test compares such traces with the oracle).
+GEON_D void theta_sincos_dw(double x, double* s, double* sl, double* c, double* cl) {
+#if defined(__HIP_DEVICE_COMPILE__) || defined(GEON_HOST_DEVICE_MATH)
+    sincos_acc_dw(x, s, sl, c, cl);
+#else
+    sincos(x, s, c);
+    *sl = 0.0;
+    *cl = 0.0;
+#endif
+}
+GEON_D void stage_sincos_dw(double th, double th0, double s0, double s0l, double c0, double c0l,
+                            double* s, double* c) {
+#if defined(__HIP_DEVICE_COMPILE__) || defined(GEON_HOST_DEVICE_MATH)
+    sincos_rot_dw(th, s0, s0l, c0, c0l, th - th0, s, c);
+#else
+    (void)th0; (void)s0; (void)s0l; (void)c0; (void)c0l;
     sincos(th, s, c);
 #endif
 }
@@ -457,24 +597,48 @@ struct Schwarzschild {
     // Shared subexpressions are computed once: 2M, r², r²f and r − 2M enter
     // several entries with the same operands, and −M/(r²f) = −(M/(r²f)).
     GEON_D static void christoffel(const geon_dmetric& P, const double* x, double* c) {
-        const double r = x[1], m = P.M;
-        const double two_m = 2.0 * m;
-        const double inv_r = 1.0 / r;
-        // 2M/r from 1/r; a non-finite r or 1/r leaves f non-finite as the
-        // IEEE division would, and the step is discarded as diverged either way
-        const double f = 1.0 - div_rn(two_m, r, inv_r);
-        // the library sincos: rays that orbit near the photon sphere amplify
-        // the ~1-ulp error of sincos_cw into their escape directions (2.6e-8
-        // rad at r = 12M against the oracle), where the library keeps them
-        // near 1e-13
         double sin_t, cos_t;
         sincos(x[2], &sin_t, &cos_t);
+        christoffel_sc(P, x, sin_t, cos_t, c);
+    }
+    // the same with sin θ and cos θ given (rk4_step hands each stage its
+    // values: theta_sincos_dw at the step's θ, stage_sincos_dw at the stages')
+    GEON_D static void christoffel_sc(const geon_dmetric& P, const double* x, double sin_t,
+                                      double cos_t, double* c) {
+        const double r = x[1], m = P.M;
+        const double two_m = 2.0 * m;
         const double r2 = r * r;
-        const double gtr = m / (r2 * f);
         const double rs = r - two_m;
+        // Every quotient is the IEEE one. Where 2^-100 < |r|, |f| < 2^100 and
+        // the host found 2^-100 ≤ M ≤ 2^100 (P.mk), all operands and
+        // quotients lie within 2^±400 and the shorter sequences give the same
+        // bits: 1/r by rcp_rn, 2M/r from it by Markstein's correction (2M is
+        // never −0), the rest by div_ns; elsewhere the divisions themselves.
+        const double ar = fabs(r);
+        const bool rok = P.mk && ar > 0x1p-100 && ar < 0x1p100;
+        double inv_r, f;
+        if (__builtin_expect(rok, 1)) {
+            inv_r = rcp_rn(r);
+            f = 1.0 - div_rn_pz(two_m, r, inv_r);
+        } else {
+            inv_r = 1.0 / r;
+            f = 1.0 - two_m / r;
+        }
+        const double af = fabs(f);
+        double gtr, gtt, cot;
+        if (__builtin_expect(rok && af > 0x1p-100 && af < 0x1p100, 1)) {
+            gtr = div_ns(m, r2 * f);
+            gtt = div_ns(m * f, r2);
+            // cos θ is never zero (π/2 is irrational) nor within 2^-60 of it
+            cot = (fabs(sin_t) < 1e-8) ? 0.0 : div_ns(cos_t, sin_t);
+        } else {
+            gtr = m / (r2 * f);
+            gtt = m * f / r2;
+            cot = (fabs(sin_t) < 1e-8) ? 0.0 : cos_t / sin_t;
+        }
         c[0 * 16 + 0 * 4 + 1] = gtr;
         c[0 * 16 + 1 * 4 + 0] = gtr;
-        c[1 * 16 + 0 * 4 + 0] = m * f / r2;
+        c[1 * 16 + 0 * 4 + 0] = gtt;
         c[1 * 16 + 1 * 4 + 1] = -gtr;
         c[1 * 16 + 2 * 4 + 2] = -rs;
         c[1 * 16 + 3 * 4 + 3] = -rs * sin_t * sin_t;
@@ -483,13 +647,20 @@ struct Schwarzschild {
         c[2 * 16 + 3 * 4 + 3] = -sin_t * cos_t;
         c[3 * 16 + 1 * 4 + 3] = inv_r;
         c[3 * 16 + 3 * 4 + 1] = inv_r;
-        const double cot = (fabs(sin_t) < 1e-8) ? 0.0 : cos_t / sin_t;
         c[3 * 16 + 2 * 4 + 3] = cot;
         c[3 * 16 + 3 * 4 + 2] = cot;
     }
     GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
         double c[64];
         christoffel(P, y, c);
+        contract<Schwarzschild>(c, y + 4, dp);
+    }
+    static constexpr bool theta_stages = true;
+    static constexpr bool theta_dw = true;
+    GEON_D static void accel_sc(const geon_dmetric& P, const double* y, double* dp, double st,
+                                double ct) {
+        double c[64];
+        christoffel_sc(P, y, st, ct, c);
         contract<Schwarzschild>(c, y + 4, dp);
     }
     static constexpr bool dep(int d) { return d == 1 || d == 2; }
@@ -605,6 +776,7 @@ struct MorrisThorne {
     }
     // rk4_step hands each stage sin θ and cos θ (stage_sincos)
     static constexpr bool theta_stages = true;
+    static constexpr bool theta_dw = false;
     GEON_D static void accel_sc(const geon_dmetric& P, const double* y, double* dp, double st,
                                 double ct) {
         double c[64];
@@ -772,7 +944,7 @@ GEON_D void rk4_step_flat(const double* y, double h, double* out) {
 // in that order. Metrics with flat regions take the folded step there.
 // Metrics whose Γ takes sin θ and cos θ as arguments (theta_stages) get
 // them from rk4_step: theta_sincos at the step's θ, stage_sincos at the
-// stages'.
+// stages' (theta_dw: their two-word forms).
 template <class MP, class = void>
 struct ThetaStages : std::false_type {};
 template <class MP>
@@ -791,18 +963,24 @@ template <class MP>
 GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, double* out) {
     double acc[8], yt[8], dp[4];
     const double hh = 0.5 * h;
-    double s0 = 0.0, c0 = 0.0;
+    double s0 = 0.0, c0 = 0.0, s0l = 0.0, c0l = 0.0;
     auto stage = [&](const double* ys) {
         if constexpr (ThetaStages<MP>::value) {
             double st, ct;
-            stage_sincos(ys[2], y[2], s0, c0, &st, &ct);
+            if constexpr (MP::theta_dw)
+                stage_sincos_dw(ys[2], y[2], s0, s0l, c0, c0l, &st, &ct);
+            else
+                stage_sincos(ys[2], y[2], s0, c0, &st, &ct);
             MP::accel_sc(P, ys, dp, st, ct);
         } else {
             MP::accel(P, ys, dp);
         }
     };
     if constexpr (ThetaStages<MP>::value) {
-        theta_sincos(y[2], &s0, &c0);
+        if constexpr (MP::theta_dw)
+            theta_sincos_dw(y[2], &s0, &s0l, &c0, &c0l);
+        else
+            theta_sincos(y[2], &s0, &c0);
         MP::accel_sc(P, y, dp, s0, c0);
     } else {
         MP::accel(P, y, dp);

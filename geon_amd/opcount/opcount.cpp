@@ -82,6 +82,7 @@ inline bool cnt_isnan(cnt_d a) { return ::isnan(a.v); }
 inline cnt_d div_rn(cnt_d a, cnt_d b, cnt_d) { ++T.div; return cnt_d(a.v / b.v); }
 inline cnt_d div_rn_pz(cnt_d a, cnt_d b, cnt_d) { ++T.div; return cnt_d(a.v / b.v); }
 inline cnt_d rcp_rn(cnt_d a) { ++T.div; return cnt_d(1.0 / a.v); }
+inline cnt_d div_ns(cnt_d a, cnt_d b) { ++T.div; return cnt_d(a.v / b.v); }
 // the kernels' own sincos is one sine and one cosine
 inline void sincos_cw(cnt_d a, cnt_d* s, cnt_d* c) { sincos(a, s, c); }
 // and its tanh one tanh

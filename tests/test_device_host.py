@@ -42,6 +42,25 @@ def hd(tmp_path_factory):
     return C.CDLL(so)
 
 
+@pytest.fixture(scope="module")
+def hd_dm(tmp_path_factory):
+    """The same host build with the GPU's own sin/cos routines where the GPU
+    takes them (GEON_HOST_DEVICE_MATH)."""
+    if shutil.which("g++") is None:
+        pytest.skip("no host compiler")
+    so = str(tmp_path_factory.mktemp("hddm") / "libhddm.so")
+    b = subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off",
+                        "-DGEON_HOST_DEVICE_MATH",
+                        "-Wno-unused-result", "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
+                        f"-I{os.path.join(ROOT, 'include')}",
+                        os.path.join(ROOT, "tests", "host", "device_host_driver.cpp"),
+                        os.path.join(ROOT, "geon_amd", "csrc", "host", "geon_runtime.cpp"),
+                        f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib", "-o", so],
+                       capture_output=True, text=True, timeout=600)
+    assert b.returncode == 0, b.stderr[-3000:]
+    return C.CDLL(so)
+
+
 def _p(a):
     return a.ctypes.data_as(C.c_void_p)
 
@@ -156,3 +175,28 @@ def test_alcubierre_curvature_team_matches_oracle(hd, O):
             assert hd.hd_curvature_alc_quad(C.byref(m), _p(x), C.c_long(len(x)), what,
                                             C.c_double(factor), _p(got)) == 0
             assert np.array_equal(got, np.asarray(ref).reshape(-1)), ("quad", what)
+
+
+@pytest.mark.parametrize("obs,esc,max_steps,w,h", [
+    ((0.0, 30.0, math.pi / 2, 0.0), 100.0, 4000, 40, 40),
+    ((0.0, 12.0, 1.3, 0.4), 80.0, 8000, 32, 24),
+])
+def test_schwarzschild_device_sincos_traces(hd_dm, O, obs, esc, max_steps, w, h):
+    """Schwarzschild traces with the GPU's sin θ / cos θ (sincos_acc_dw at each
+    step, sincos_rot_dw at its stages), run on the host: classes and step
+    counts equal to the oracle's on every ray and escape directions within
+    1e-9 rad, most final states equal bit for bit."""
+    m = O.metric(O.SCHWARZSCHILD, horizon_pad=1.005, escape_radius=esc)
+    y0 = O.camera_rays(m, O.camera(obs, 60.0, w, h))
+    ref = O.trace_rays(m, y0, 0.05, max_steps)
+    cls, steps, Y = _trace(hd_dm, m, y0, 0.05, max_steps)
+    assert np.array_equal(cls, ref["cls"]) and np.array_equal(steps, ref["steps"])
+    esc_ok = (cls == O.ESCAPED) & (np.abs(ref["y"][:, 1]) < 10 * esc)
+    d = np.array([O.asymptotic_direction(m, y[:4], y[4:]) for y in Y[esc_ok]])
+    from parity_util import angle
+    a = angle(d, np.asarray(ref["dir"])[esc_ok])
+    same = np.mean(np.all(Y == np.asarray(ref["y"]), axis=1))
+    print(f"device sincos on the host, obs {obs}: max escape-angle difference {a.max():.3e} "
+          f"rad over {esc_ok.sum()} rays, final states bit-equal {same:.3f}")
+    assert a.max() <= 1e-9
+    assert same >= 0.75
