@@ -116,6 +116,7 @@ static geon_sched_cfg_t resolve_cfg(geon_ctx* c, int kind, int prec, const geon_
     if (r.waves_per_simd < 1) r.waves_per_simd = 1;
     if (r.waves_per_simd > 8) r.waves_per_simd = 8;
     if (r.refill < 1) r.refill = 1;
+    if (r.refill > 65536) r.refill = 65536;   // the kernels count a chunk's steps in 32 bits
     return r;
 }
 

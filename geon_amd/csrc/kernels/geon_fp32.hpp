@@ -147,22 +147,23 @@ struct SchwarzschildF {
     static constexpr bool uses_theta = true;
     static constexpr int flat_test = 0;
     // 1/r, 1/(r − 2M) and cot θ from one reciprocal of r·(r − 2M)·sin θ
-    // (reciprocals issue at a quarter of the VALU rate)
+    // (reciprocals issue at a quarter of the VALU rate). No pole guard: a
+    // stage this close to the pole has |sin θ| below the pole band, so the
+    // ray goes to the FP64 path whatever this step gives (at sin θ = 0 it
+    // gives NaN, which ends the ray as diverged).
     GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
         const float r = y[1];
         const float pt = y[4], pr = y[5], pth = y[6], pph = y[7];
-        const bool pole = fabsf(s) < 1e-8f;
-        const float se = pole ? 1.0f : s;
         const float rf = __builtin_fmaf(-2.0f, P.M, r);   // r − 2M = r·f
         const float rr = r * rf;
-        const float q = frcp(rr * se);
-        const float inv_r = (rf * se) * q;
-        const float inv_rr = se * q;                 // 1/(r(r − 2M))
+        const float q = frcp(rr * s);
+        const float inv_r = (rf * s) * q;
+        const float inv_rr = s * q;                  // 1/(r(r − 2M))
         const float f = rf * inv_r;
         const float a = (P.M * inv_r) * inv_r;       // M/r²
         const float gtr = P.M * inv_rr;              // Γ^t_tr = M/(r²f)
         const float grtt = a * f;                    // Γ^r_tt
-        const float cot = pole ? 0.0f : (c * rr) * q;
+        const float cot = (c * rr) * q;
         const float pph2 = pph * pph;
         dp[0] = ((-2.0f * gtr) * pt) * pr;
         // −(Γ^r_tt pt² − M/(r²f)·pr²) + (r − 2M)(pθ² + sin²θ pφ²)
@@ -181,10 +182,9 @@ struct SchwarzschildF {
     }
     GEON_D static int terminate(const FParams& P, const float* x) {
         const float r = x[1];
+        if (r >= P.cap_r && r <= P.esc_r) return RUNNING;   // NaN fails both
         if (!__builtin_isfinite(r)) return DIVERGED;
-        if (r < P.cap_r) return CAPTURED;
-        if (r > P.esc_r) return ESCAPED;
-        return RUNNING;
+        return (r < P.cap_r) ? CAPTURED : ESCAPED;
     }
     // Capture is decided by FP64 too: whether the reference's fixed step
     // crosses r = 2.01M smoothly or has a stage land on r ≈ 2M (and fling the
@@ -201,16 +201,15 @@ struct SchwarzschildF {
 struct MorrisThorneF {
     static constexpr bool uses_theta = true;
     static constexpr int flat_test = 0;
+    // no pole guard, as SchwarzschildF::accel
     GEON_D static void accel(const FParams& P, const float* y, float* dp, float s, float c) {
         const float l = y[1];
         const float pl = y[5], pth = y[6], pph = y[7];
-        const bool pole = fabsf(s) < 1e-8f;
-        const float se = pole ? 1.0f : s;
         const float r2 = __builtin_fmaf(l, l, P.b0sq);
-        const float q = frcp(r2 * se);               // 1/r² and cot θ from one reciprocal
-        const float inv_r2 = se * q;
+        const float q = frcp(r2 * s);                // 1/r² and cot θ from one reciprocal
+        const float inv_r2 = s * q;
         const float lr = l * inv_r2;                 // Γ^θ_lθ = Γ^φ_lφ
-        const float cot = pole ? 0.0f : (c * r2) * q;
+        const float cot = (c * r2) * q;
         const float pph2 = pph * pph;
         dp[0] = 0.0f;
         // −(Γ^l_θθ pθ² + Γ^l_φφ pφ²)
@@ -225,8 +224,8 @@ struct MorrisThorneF {
     }
     GEON_D static int terminate(const FParams& P, const float* x) {
         const float l = x[1];
-        if (!__builtin_isfinite(l)) return DIVERGED;
-        return (fabsf(l) > P.esc_l) ? ESCAPED : RUNNING;
+        if (fabsf(l) <= P.esc_l) return RUNNING;   // NaN fails it
+        return __builtin_isfinite(l) ? ESCAPED : DIVERGED;
     }
     GEON_D static bool suspicious(const FParams& P, int cls, const float* y, float min_s, float min_r) {
         if (cls == DIVERGED) return true;
@@ -350,8 +349,8 @@ GEON_D void rotate_sc(float s, float c, float theta_new, float d, float& so, flo
 // FP32 RK4 step with the reference tableau. (s, c) carry sin/cos of y[2] in
 // and of out[2] out: one accurate sincosf per step, the stage angles θ + δ
 // come from rotations by the small stage increments δ. p^t is re-projected
-// at the end of the step. Returns |sin θ| of the new state (1 for Cartesian
-// charts).
+// at the end of the step. Returns the smallest |sin θ| of the new state and
+// the step's stages (1 for Cartesian charts).
 // A run of straight-line (Γ = 0) FP32 steps: the steps still known to be
 // flat, the steps taken since the run began and the position it began at.
 // Step j of a run puts the ray at a + (j·h)·p, one rounding from the exact
@@ -413,7 +412,11 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
         yt[i] = __builtin_fmaf(hh, y[4 + i], y[i]);
         yt[4 + i] = __builtin_fmaf(hh, dp[i], y[4 + i]);
     }
-    if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], hh * y[6], ss, cs);
+    float smin = 1.0f;   // the smallest stage |sin θ| (the pole band looks at stages too)
+    if constexpr (MF::uses_theta) {
+        rotate_sc(s, c, yt[2], hh * y[6], ss, cs);
+        smin = fabsf(ss);
+    }
     MF::accel(P, yt, dp, ss, cs);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -423,7 +426,10 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
         yt[i] = __builtin_fmaf(hh, kx, y[i]);
         yt[4 + i] = __builtin_fmaf(hh, dp[i], y[4 + i]);
     }
-    if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], yt[2] - y[2], ss, cs);
+    if constexpr (MF::uses_theta) {
+        rotate_sc(s, c, yt[2], yt[2] - y[2], ss, cs);
+        smin = fminf(smin, fabsf(ss));
+    }
     MF::accel(P, yt, dp, ss, cs);
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -433,7 +439,10 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
         yt[i] = __builtin_fmaf(h, kx, y[i]);
         yt[4 + i] = __builtin_fmaf(h, dp[i], y[4 + i]);
     }
-    if constexpr (MF::uses_theta) rotate_sc(s, c, yt[2], yt[2] - y[2], ss, cs);
+    if constexpr (MF::uses_theta) {
+        rotate_sc(s, c, yt[2], yt[2] - y[2], ss, cs);
+        smin = fminf(smin, fabsf(ss));
+    }
     MF::accel(P, yt, dp, ss, cs);
     const float sixth = h * (1.0f / 6.0f);
 #pragma unroll
@@ -446,7 +455,7 @@ GEON_D float rk4_step_f(const FParams& P, const float* y, float h, float* out, f
     if constexpr (MF::uses_theta) {
         sincosf_cw(out[2], &s, &c);
         MF::reproject(P, out, s);
-        return fabsf(s);
+        return fminf(fabsf(s), smin);
     } else {
         MF::reproject(P, out);
         return 2.0f;   // a step that evaluated Γ (flat steps return 1)

@@ -262,6 +262,7 @@ __device__ void sched_run(const geon_sched_args& A) {
     typename PR::Run fr[RPL];       // flat-step runs
     Params P[RPL];
     unsigned long long executed = 0;
+    uint32_t ex32 = 0;   // RK4 steps since the last refill point (at most refill × RPL)
 
     auto activate = [&](int r, unsigned long long gid, uint32_t steps_done, bool resume) {
         const uint32_t k = find_job(A.job_base, A.n_jobs, gid);
@@ -337,6 +338,8 @@ __device__ void sched_run(const geon_sched_args& A) {
         // ---- refill point (all lanes of the wave converged here) ----
         wave_add(done_p, done_n);
         done_n = 0;
+        executed += ex32;
+        ex32 = 0;
         // A wave claims when a quarter of its slots are empty (or all are):
         // one atomicAdd per claim. A claim may run past `end`; positions at or
         // beyond it are simply not used, and between serving quanta the host
@@ -426,13 +429,14 @@ __device__ void sched_run(const geon_sched_args& A) {
                 cs[r] = c2;
                 PR::track(min_s[r], min_r[r], asn, yn);
                 ++n[r];
-                ++executed;
+                ++ex32;
             }
             if (!live) break;
         }
         if (A.quantum) budget -= chunk;
     }
     wave_add(done_p, done_n);
+    executed += ex32;
 
     // park rays still in flight
     uint32_t in_flight = 0;

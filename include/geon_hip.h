@@ -315,7 +315,7 @@ typedef struct geon_sched_cfg {
     uint32_t block;           /* threads per workgroup (64, 128 or 256)        */
     uint32_t waves_per_simd;  /* resident waves per SIMD the grid is sized for */
     uint32_t rays_per_lane;   /* 1 or 2 rays integrated per lane               */
-    uint32_t refill;          /* RK4 steps between refill points               */
+    uint32_t refill;          /* RK4 steps between refill points (1..65536)    */
 } geon_sched_cfg_t;
 
 typedef struct geon_batch geon_batch_t;
