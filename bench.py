@@ -369,35 +369,52 @@ def tune_kinds(geon, ctx):
     _TUNED.add(id(ctx))
 
 
+def _run_batch(geon, ctx, torch, stream, ev, frames):
+    """One batch launch over `frames`; returns (device outputs, stats)."""
+    bt = geon.Batch(ctx)
+    o = []
+    for m, cam, cfg in frames:
+        n = cam.width * cam.height
+        cls = torch.empty(n, dtype=torch.int32, device="cuda")
+        d = torch.empty(3 * n, dtype=torch.float64, device="cuda")
+        o.append((type(m).__name__, cls, d))
+    ids = [bt.add_frame(m, cam, cfg, 0, c.data_ptr(), d.data_ptr())
+           for (m, cam, cfg), (_, c, d) in zip(frames, o)]
+    torch.cuda.synchronize()
+    a, b = ev(), ev()
+    a.record(stream)
+    bt.run(stream_ptr=stream.cuda_stream)
+    b.record(stream)
+    b.synchronize()
+    ms = a.elapsed_time(b)
+    steps = sum(int(bt.frame_stats(i)["ray_steps"]) for i in ids)
+    tot = bt.totals()
+    bt.close()
+    return o, {"ms": round(ms, 2), "ray_steps": steps,
+               "ray_steps_per_s": steps / (ms * 1e-3), "fp64_rays": tot["fp64_rays"]}
+
+
 def ragged_batch(geon, ctx, torch, stream, ev):
     from geon_amd import workloads
     res = {}
     outs = {}
     tune_kinds(geon, ctx)
     for label, prec in (("fp64", geon.PREC_FP64), ("mixed", geon.PREC_MIXED)):
-        frames = workloads.ragged_frames(precision=prec)
-        bt = geon.Batch(ctx)
-        o = []
-        for m, cam, cfg in frames:
-            n = cam.width * cam.height
-            cls = torch.empty(n, dtype=torch.int32, device="cuda")
-            d = torch.empty(3 * n, dtype=torch.float64, device="cuda")
-            o.append((type(m).__name__, cls, d))
-        ids = [bt.add_frame(m, cam, cfg, 0, c.data_ptr(), d.data_ptr())
-               for (m, cam, cfg), (_, c, d) in zip(frames, o)]
-        torch.cuda.synchronize()
-        a, b = ev(), ev()
-        a.record(stream)
-        bt.run(stream_ptr=stream.cuda_stream)
-        b.record(stream)
-        b.synchronize()
-        ms = a.elapsed_time(b)
-        steps = sum(int(bt.frame_stats(i)["ray_steps"]) for i in ids)
-        tot = bt.totals()
-        bt.close()
-        outs[label] = o
-        res[label] = {"ms": round(ms, 2), "ray_steps": steps,
-                      "ray_steps_per_s": steps / (ms * 1e-3), "fp64_rays": tot["fp64_rays"]}
+        outs[label], res[label] = _run_batch(geon, ctx, torch, stream, ev,
+                                             workloads.ragged_frames(precision=prec))
+    # each metric kind's frames alone, FP64 and FP32-mixed
+    per_kind = {}
+    for kind in ("Schwarzschild", "MorrisThorne", "Alcubierre"):
+        row = {}
+        for label, prec in (("fp64", geon.PREC_FP64), ("mixed", geon.PREC_MIXED)):
+            fr = [f for f in workloads.ragged_frames(precision=prec) if type(f[0]).__name__ == kind]
+            _, st = _run_batch(geon, ctx, torch, stream, ev, fr)
+            row["frames"] = len(fr)
+            row[label + "_ms"] = st["ms"]
+            row[label + "_fp64_rays"] = st["fp64_rays"]
+        row["speedup_mixed_vs_fp64"] = row["fp64_ms"] / row["mixed_ms"]
+        per_kind[kind] = row
+    res["per_kind_alone"] = per_kind
     per, allk = {}, _acc()
     for (kind, c64, d64), (_, cm, dm) in zip(outs["fp64"], outs["mixed"]):
         _mixed_vs_fp64(torch, c64, d64, cm, dm, per.setdefault(kind, _acc()))

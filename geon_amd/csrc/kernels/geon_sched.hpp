@@ -393,43 +393,41 @@ __device__ void sched_run(const geon_sched_args& A) {
             for (int r = 0; r < RPL; ++r) {
                 if (!act[r]) continue;
                 live = true;
-                if (n[r] >= maxs[r]) {
-                    finish(r, MAXSTEPS);
-                    continue;
-                }
-                int t = PR::terminate(P[r], y[r]);
+                // the ray's outcome this iteration: RUNNING, or the class it
+                // ends with (max steps first, then the termination test, then
+                // the step's NaN guard, which keeps the pre-step state)
+                int t = (n[r] >= maxs[r]) ? MAXSTEPS : PR::terminate(P[r], y[r]);
                 if (t == RUNNING && early[r]) {
                     if constexpr (std::is_same<MP, Schwarzschild>::value) {
                         const geon_dmetric& Pm = A.jobs[jb[r]].F.m;
                         t = photon_sphere_decision(Pm, (double)y[r][1], (double)y[r][5]);
                     }
                 }
-                if (t != RUNNING) {
-                    finish(r, t);
-                    continue;
-                }
-                if constexpr (MP::flat_test != 0) {
-                    const uint32_t m = PR::span(P[r], y[r], h[r], fr[r], maxs[r] - n[r]);
-                    if (m > 0) {
+                if (t == RUNNING) {
+                    uint32_t m = 0;
+                    if constexpr (MP::flat_test != 0) {
+                        m = PR::span(P[r], y[r], h[r], fr[r], maxs[r] - n[r]);
                         n[r] += m;
                         executed += m;
-                        continue;
+                    }
+                    if (m == 0) {
+                        T yn[8];
+                        float s2 = sn[r], c2 = cs[r];
+                        const float asn = PR::step(P[r], y[r], h[r], yn, s2, c2, fr[r]);
+                        if (PR::finite(yn)) {
+#pragma unroll
+                            for (int c = 0; c < 8; ++c) y[r][c] = yn[c];
+                            sn[r] = s2;
+                            cs[r] = c2;
+                            PR::track(min_s[r], min_r[r], asn, yn);
+                            ++n[r];
+                            ++ex32;
+                        } else {
+                            t = DIVERGED;
+                        }
                     }
                 }
-                T yn[8];
-                float s2 = sn[r], c2 = cs[r];
-                const float asn = PR::step(P[r], y[r], h[r], yn, s2, c2, fr[r]);
-                if (!PR::finite(yn)) {
-                    finish(r, DIVERGED);
-                    continue;
-                }
-#pragma unroll
-                for (int c = 0; c < 8; ++c) y[r][c] = yn[c];
-                sn[r] = s2;
-                cs[r] = c2;
-                PR::track(min_s[r], min_r[r], asn, yn);
-                ++n[r];
-                ++ex32;
+                if (t != RUNNING) finish(r, t);
             }
             if (!live) break;
         }
