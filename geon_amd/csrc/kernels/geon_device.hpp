@@ -1196,6 +1196,48 @@ GEON_D int trace(const geon_dmetric& P, double* y, double h, uint32_t max_steps,
     FlatRun fr;
     uint32_t n = 0, ng = 0;
     int cls = MAXSTEPS;
+    if constexpr (MP::flat_test == 0) {
+        // Two steps per iteration, the state alternating between ya and yb:
+        // the same steps as the loop below, without copying the state back
+        // each step (the copy was a register shuffle of the whole state).
+        double ya[8], yb[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ya[i] = y[i];
+        bool in_b = false;   // the current state is yb
+        for (;;) {
+            if (n >= max_steps) break;
+            int t = MP::terminate(P, ya);
+            if (t != RUNNING) {
+                cls = t;
+                break;
+            }
+            rk4_step_full<MP>(P, ya, h, yb);
+            if (!finite8(yb)) {
+                cls = DIVERGED;
+                break;
+            }
+            ++n;
+            in_b = true;
+            if (n >= max_steps) break;
+            t = MP::terminate(P, yb);
+            if (t != RUNNING) {
+                cls = t;
+                break;
+            }
+            rk4_step_full<MP>(P, yb, h, ya);
+            if (!finite8(ya)) {
+                cls = DIVERGED;
+                break;
+            }
+            ++n;
+            in_b = false;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) y[i] = in_b ? yb[i] : ya[i];
+        *steps = n;
+        if (gamma_steps) *gamma_steps = n;
+        return cls;
+    }
     while (n < max_steps) {
         const int t = MP::terminate(P, y);
         if (t != RUNNING) {
