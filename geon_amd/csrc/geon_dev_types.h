@@ -26,7 +26,7 @@ struct geon_dmetric {
     double  fd_sin_h, fd_cos_h;   // sin h, cos h: θ ± h rotations (Morris–Thorne)
     double  inv_denom;   // 1.0 / denom (Alcubierre)
     int32_t mk;          // 1: quotients may take the shorter sequences (Alcubierre: by 2h and denom; Schwarzschild: M in range; see make_dmetric)
-    int32_t pad_;
+    int32_t shape_exp;   // Alcubierre: 1 if a curvature stencil's tanh arguments lie within 1e-3 of its centre's
 };
 
 struct geon_dframe {

@@ -154,6 +154,11 @@ int make_dmetric(const geon_metric_t* m, geon_dmetric* d) {
         // 0 or at least an ulp of 1.
         d->mk = (d->s > 0.0 && d->denom >= 1e-3 && d->denom <= 2.0 && fabs(d->v) <= 1e6 &&
                  d->fd_two_h >= 1e-150 && d->fd_two_h <= 1e150) ? 1 : 0;
+        // A curvature stencil point is at most two FD steps from the point
+        // (|Δr_s| ≤ 2h), so its tanh arguments differ from the centre's by
+        // at most 2h/|s|: the kernels expand tanh about the centre where that
+        // is ≤ 1e-3 (fifth-order Taylor polynomial, remainder below 1e-18).
+        d->shape_exp = (d->mk && 2.0 * fabs(m->fd_step) <= 1e-3 * fabs(d->s)) ? 1 : 0;
     }
     // Schwarzschild's quotients take the shorter sequences where M, r and
     // f = 1 − 2M/r lie within 2^±100 (the kernels test r and f)

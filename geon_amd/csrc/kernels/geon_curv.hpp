@@ -15,7 +15,8 @@
 // x + s2·h e_a, and x ± h e_d serves both as a shift of x and as the centre
 // of stencil point ± d. The team therefore
 //   A. evaluates the shape function f (alcubierre.rs:33-44) at the 31
-//      distinct points, 4 rounds of 8 lanes, into LDS;
+//      distinct points, 4 rounds of 8 lanes, into LDS (on the GPU from the
+//      expansion of its two tanh about the point, Alcubierre::shape_exp);
 //   B. forms, on lane t < 7, Γ at stencil point t from those f values with
 //      the generic formula (g from f, g⁻¹, ∂g by central differences, Γ) and
 //      writes its 30 nonzero entries to LDS;
@@ -129,11 +130,26 @@ struct AlcStencil {
     }
 };
 
-// phase A: the shape value f at metric-stencil slot j of point x
-GEON_D double curv_alc_point_f(const geon_dmetric& P, const double* x, int j) {
+// phase A: the shape value f at metric-stencil slot j of point x; with E
+// (the expansion about x, Alcubierre::shape_exp), from the expansion
+GEON_D double curv_alc_point_f(const geon_dmetric& P, const double* x, int j,
+                               const Alcubierre::ShapeExp* E = nullptr) {
     double xe[4];
     AlcStencil::point(P, x, j, xe);
-    return Alcubierre::shape<true>(P, Alcubierre::rs(P, xe));
+    const double r = Alcubierre::rs(P, xe);
+    return E ? Alcubierre::shape_near(P, *E, r) : Alcubierre::shape<true>(P, r);
+}
+// Whether this build's phase A expands the shape function about the point:
+// on the GPU where the stencil allows it (P.shape_exp); the host build
+// evaluates every tanh, as the reference, unless GEON_HOST_DEVICE_MATH asks
+// for the GPU's arithmetic.
+GEON_D bool curv_shape_exp(const geon_dmetric& P) {
+#if defined(__HIP_DEVICE_COMPILE__) || defined(GEON_HOST_DEVICE_MATH)
+    return P.shape_exp != 0;
+#else
+    (void)P;
+    return false;
+#endif
 }
 
 // phase B: Γ at Γ-stencil point t (< 7) from the staged f values, as
@@ -327,7 +343,7 @@ struct AlcQuad {
     // y², z² at the point; lane 3 of round 1 repeats x (slot 0).
     template <int R>
     GEON_D static void shape_round(const geon_dmetric& P, const double* x, const double* sq, int t,
-                                   double* pt) {
+                                   double* pt, const Alcubierre::ShapeExp* E = nullptr) {
         const double h = P.fd_h;
         const double hA = (t & 2) ? -h : h, hB = (t & 1) ? -h : h;
         double s[3] = {sq[0], sq[1], sq[2]};
@@ -355,7 +371,8 @@ struct AlcQuad {
             s[hi - 1] = shifted(hi, x[hi] + hB);
             j = 19 + 4 * p + t;
         }
-        pt[j] = Alcubierre::shape<true>(P, sqrt(s[0] + s[1] + s[2]));
+        const double r = sqrt(s[0] + s[1] + s[2]);
+        pt[j] = E ? Alcubierre::shape_near(P, *E, r) : Alcubierre::shape<true>(P, r);
     }
     // the squares of the unshifted coordinates (Alcubierre::rs)
     GEON_D static void base_squares(const geon_dmetric& P, const double* x, double* sq) {

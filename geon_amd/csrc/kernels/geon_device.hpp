@@ -841,6 +841,52 @@ struct Alcubierre {
         // tp ≥ tm, so the difference is never −0
         return P.mk ? div_rn_pz(tp - tm, P.denom, P.inv_denom) : (tp - tm) / P.denom;
     }
+    // The shape function at points r_s close to a centre r0 (curvature
+    // stencils, P.shape_exp): both tanh expanded about their values at r0,
+    // tanh(u0 + δ) = t + Σ_{k=1..5} c_k δ^k with t = tanh(u0), s2 = 1 − t²,
+    // c1 = s2, c2 = −t·s2, c3 = s2(t² − 1/3), c4 = t·s2(2/3 − t²),
+    // c5 = s2(2/15 − t² + t⁴) (the derivatives of tanh in terms of tanh),
+    // |δ| ≤ 1e-3: the remainder is below 1e-18. The centre's tanh carries
+    // its own last-bit error into every stencil point alike, so the
+    // differences that the central differences take are formed from the
+    // small, accurately rounded corrections.
+    struct ShapeExp {
+        double up, um;          // (r0 ± R)/s
+        double tp, tm;          // their tanh
+        double cp[5], cm[5];
+    };
+    GEON_D static double shape_arg(const geon_dmetric& P, double v) {
+        return P.s_pow2 ? v * P.inv_s : v / P.s;
+    }
+    GEON_D static void tanh_coeffs(double t, double* c) {
+        const double t2 = t * t;
+        const double s2 = (1.0 - t) * (1.0 + t);
+        c[0] = s2;
+        c[1] = -t * s2;
+        c[2] = s2 * (t2 - 0x1.5555555555555p-2);                    // 1/3
+        c[3] = (t * s2) * (0x1.5555555555555p-1 - t2);              // 2/3
+        c[4] = s2 * __builtin_fma(t2, t2, 0x1.1111111111111p-3 - t2);   // 2/15
+    }
+    GEON_D static double tanh_near(double t, const double* c, double d) {
+        double q = __builtin_fma(c[4], d, c[3]);
+        q = __builtin_fma(q, d, c[2]);
+        q = __builtin_fma(q, d, c[1]);
+        q = __builtin_fma(q, d, c[0]);
+        return __builtin_fma(q, d, t);
+    }
+    GEON_D static void shape_exp(const geon_dmetric& P, double r0, ShapeExp& E) {
+        E.up = shape_arg(P, r0 + P.R);
+        E.um = shape_arg(P, r0 - P.R);
+        E.tp = shape_tanh<true>(E.up);
+        E.tm = shape_tanh<true>(E.um);
+        tanh_coeffs(E.tp, E.cp);
+        tanh_coeffs(E.tm, E.cm);
+    }
+    GEON_D static double shape_near(const geon_dmetric& P, const ShapeExp& E, double r_s) {
+        const double tp = tanh_near(E.tp, E.cp, shape_arg(P, r_s + P.R) - E.up);
+        const double tm = tanh_near(E.tm, E.cm, shape_arg(P, r_s - P.R) - E.um);
+        return P.mk ? div_rn_pz(tp - tm, P.denom, P.inv_denom) : (tp - tm) / P.denom;
+    }
     // g from the shape value f at the point (alcubierre.rs:48-63)
     GEON_D static void g_of_f(const geon_dmetric& P, double f, double* g) {
         const double v = P.v;

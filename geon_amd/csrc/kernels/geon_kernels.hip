@@ -473,10 +473,13 @@ extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc(
         x[3] = xs[3 * n + k];
     }
     // A: the 31 distinct metric points, 4 rounds of the 8 lanes
+    Alcubierre::ShapeExp E;
+    const bool ex = curv_shape_exp(P);
+    if (ex) Alcubierre::shape_exp(P, Alcubierre::rs(P, x), E);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int j = t + GEON_CURV_TEAM * r;
-        if (valid && j < AlcStencil::kUsed) s_f[q][j] = curv_alc_point_f(P, x, j);
+        if (valid && j < AlcStencil::kUsed) s_f[q][j] = curv_alc_point_f(P, x, j, ex ? &E : nullptr);
     }
     __syncthreads();
     // B: Γ at the seven stencil points
@@ -552,7 +555,10 @@ extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc_ricci(
     // A: the 31 distinct metric points, 8 rounds of the 4 lanes
     double sq[3];
     AlcQuad::base_squares(P, x, sq);
-    sfor<8>([&](auto R) { AlcQuad::shape_round<CI(R)>(P, x, sq, t, pt); });
+    Alcubierre::ShapeExp E;
+    const bool ex = curv_shape_exp(P);
+    if (ex) Alcubierre::shape_exp(P, sqrt(sq[0] + sq[1] + sq[2]), E);
+    sfor<8>([&](auto R) { AlcQuad::shape_round<CI(R)>(P, x, sq, t, pt, ex ? &E : nullptr); });
     __syncthreads();
     // B: Γ at x ± h e_x, x ± h e_y (Γ-stencil points 1-4), then x ± h e_z
     // (5, 6) on lanes 0-1 and x (0) on lanes 2-3

@@ -63,10 +63,13 @@ extern "C" int hd_curvature_alc_team(const geon_metric_t* m, const double* x, lo
     for (long k = 0; k < n; ++k) {
         const double* xk = x + 4 * k;
         double sf[AlcStencil::kRow], row[CS::kRow], ric[17];
+        Alcubierre::ShapeExp E;
+        const bool ex = curv_shape_exp(P);   // the GPU's expansion (GEON_HOST_DEVICE_MATH)
+        if (ex) Alcubierre::shape_exp(P, Alcubierre::rs(P, xk), E);
         for (int t = 0; t < GEON_CURV_TEAM; ++t)   // phase A
             for (int r = 0; r < 4; ++r) {
                 const int j = t + GEON_CURV_TEAM * r;
-                if (j < AlcStencil::kUsed) sf[j] = curv_alc_point_f(P, xk, j);
+                if (j < AlcStencil::kUsed) sf[j] = curv_alc_point_f(P, xk, j, ex ? &E : nullptr);
             }
         for (int t = 0; t < CS::kPoints; ++t) curv_alc_gamma(P, sf, t, &row[t * CS::kNNZ]);
         if (what == 0) {
@@ -111,15 +114,19 @@ extern "C" int hd_curvature_alc_quad(const geon_metric_t* m, const double* x, lo
         for (int i = 0; i < AlcQuad::stride(); ++i) pt[i] = -1.0;   // unread slots show up
         double sq[3];
         AlcQuad::base_squares(P, xk, sq);
+        Alcubierre::ShapeExp E;
+        const bool ex = curv_shape_exp(P);   // the GPU's expansion (GEON_HOST_DEVICE_MATH)
+        if (ex) Alcubierre::shape_exp(P, sqrt(sq[0] + sq[1] + sq[2]), E);
+        const Alcubierre::ShapeExp* Ep = ex ? &E : nullptr;
         for (int t = 0; t < 4; ++t) {
-            AlcQuad::shape_round<0>(P, xk, sq, t, pt);
-            AlcQuad::shape_round<1>(P, xk, sq, t, pt);
-            AlcQuad::shape_round<2>(P, xk, sq, t, pt);
-            AlcQuad::shape_round<3>(P, xk, sq, t, pt);
-            AlcQuad::shape_round<4>(P, xk, sq, t, pt);
-            AlcQuad::shape_round<5>(P, xk, sq, t, pt);
-            AlcQuad::shape_round<6>(P, xk, sq, t, pt);
-            AlcQuad::shape_round<7>(P, xk, sq, t, pt);
+            AlcQuad::shape_round<0>(P, xk, sq, t, pt, Ep);
+            AlcQuad::shape_round<1>(P, xk, sq, t, pt, Ep);
+            AlcQuad::shape_round<2>(P, xk, sq, t, pt, Ep);
+            AlcQuad::shape_round<3>(P, xk, sq, t, pt, Ep);
+            AlcQuad::shape_round<4>(P, xk, sq, t, pt, Ep);
+            AlcQuad::shape_round<5>(P, xk, sq, t, pt, Ep);
+            AlcQuad::shape_round<6>(P, xk, sq, t, pt, Ep);
+            AlcQuad::shape_round<7>(P, xk, sq, t, pt, Ep);
         }
         double gs[7][30];
         for (int t = 0; t < 7; ++t) curv_alc_gamma(P, pt, t, gs[t]);
