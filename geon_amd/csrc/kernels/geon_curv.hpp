@@ -136,8 +136,8 @@ GEON_D double curv_alc_point_f(const geon_dmetric& P, const double* x, int j,
                                const Alcubierre::ShapeExp* E = nullptr) {
     double xe[4];
     AlcStencil::point(P, x, j, xe);
-    const double r = Alcubierre::rs(P, xe);
-    return E ? Alcubierre::shape_near(P, *E, r) : Alcubierre::shape<true>(P, r);
+    if (E) return Alcubierre::shape_near_sq(P, *E, Alcubierre::rs2(P, xe));
+    return Alcubierre::shape<true>(P, Alcubierre::rs(P, xe));
 }
 // Whether this build's phase A expands the shape function about the point:
 // on the GPU where the stencil allows it (P.shape_exp); the host build
@@ -371,8 +371,8 @@ struct AlcQuad {
             s[hi - 1] = shifted(hi, x[hi] + hB);
             j = 19 + 4 * p + t;
         }
-        const double r = sqrt(s[0] + s[1] + s[2]);
-        pt[j] = E ? Alcubierre::shape_near(P, *E, r) : Alcubierre::shape<true>(P, r);
+        const double S = s[0] + s[1] + s[2];
+        pt[j] = E ? Alcubierre::shape_near_sq(P, *E, S) : Alcubierre::shape<true>(P, sqrt(S));
     }
     // the squares of the unshifted coordinates (Alcubierre::rs)
     GEON_D static void base_squares(const geon_dmetric& P, const double* x, double* sq) {

@@ -823,10 +823,11 @@ struct Alcubierre {
     static constexpr bool ginv_nz(int m, int s) { return g_nz(m, s); }
     static constexpr bool gam_nz(int mu, int a, int b) { return FD<Alcubierre>::gam_nz(mu, a, b); }
 
-    GEON_D static double rs(const geon_dmetric& P, const double* x) {
+    GEON_D static double rs2(const geon_dmetric& P, const double* x) {
         const double dx = x[1] - P.ship_x, dy = x[2], dz = x[3];
-        return sqrt(dx * dx + dy * dy + dz * dz);
+        return dx * dx + dy * dy + dz * dz;
     }
+    GEON_D static double rs(const geon_dmetric& P, const double* x) { return sqrt(rs2(P, x)); }
     // alcubierre.rs:33-44 — denom = 2·tanh(R/s) precomputed on the host
     template <bool INLINE_TANH = false>
     GEON_D static double shape(const geon_dmetric& P, double r_s) {
@@ -851,6 +852,7 @@ struct Alcubierre {
     // differences that the central differences take are formed from the
     // small, accurately rounded corrections.
     struct ShapeExp {
+        double S0, r0, inv_S0;  // |x − x_s|² at the centre, its square root, 1/S0
         double up, um;          // (r0 ± R)/s
         double tp, tm;          // their tanh
         double cp[5], cm[5];
@@ -874,18 +876,41 @@ struct Alcubierre {
         q = __builtin_fma(q, d, c[0]);
         return __builtin_fma(q, d, t);
     }
-    GEON_D static void shape_exp(const geon_dmetric& P, double r0, ShapeExp& E) {
-        E.up = shape_arg(P, r0 + P.R);
-        E.um = shape_arg(P, r0 - P.R);
+    // S0 = dx² + dy² + dz² at the centre, as rs() sums it
+    GEON_D static void shape_exp(const geon_dmetric& P, double S0, ShapeExp& E) {
+        E.S0 = S0;
+        E.r0 = sqrt(S0);
+        E.inv_S0 = 1.0 / S0;
+        E.up = shape_arg(P, E.r0 + P.R);
+        E.um = shape_arg(P, E.r0 - P.R);
         E.tp = shape_tanh<true>(E.up);
         E.tm = shape_tanh<true>(E.um);
         tanh_coeffs(E.tp, E.cp);
         tanh_coeffs(E.tm, E.cm);
     }
+    GEON_D static double shape_of_tanh(const geon_dmetric& P, double tp, double tm) {
+        return P.mk ? div_rn_pz(tp - tm, P.denom, P.inv_denom) : (tp - tm) / P.denom;
+    }
     GEON_D static double shape_near(const geon_dmetric& P, const ShapeExp& E, double r_s) {
         const double tp = tanh_near(E.tp, E.cp, shape_arg(P, r_s + P.R) - E.up);
         const double tm = tanh_near(E.tm, E.cm, shape_arg(P, r_s - P.R) - E.um);
-        return P.mk ? div_rn_pz(tp - tm, P.denom, P.inv_denom) : (tp - tm) / P.denom;
+        return shape_of_tanh(P, tp, tm);
+    }
+    // The same from the stencil point's S = dx² + dy² + dz²: r_s − r0 =
+    // r0·(√(1 + ε) − 1) with ε = (S − S0)/S0 from its Taylor series to ε⁵
+    // (|ε| ≤ 1e-3: remainder below 2e-20 relative), so neither the square
+    // root nor the tanh arguments are formed; both tanh move by (r_s − r0)/s.
+    // Points with |ε| > 1e-3 (next to the bubble's centre) take the square
+    // root.
+    GEON_D static double shape_near_sq(const geon_dmetric& P, const ShapeExp& E, double S) {
+        const double eps = (S - E.S0) * E.inv_S0;
+        if (!(fabs(eps) <= 1e-3)) return shape_near(P, E, sqrt(S));
+        double q = __builtin_fma(eps, 0x1.cp-6, -0x1.4p-5);      //  7/256, −5/128
+        q = __builtin_fma(q, eps, 0.0625);                          //  1/16
+        q = __builtin_fma(q, eps, -0.125);                          // −1/8
+        q = __builtin_fma(q, eps, 0.5);                             //  1/2
+        const double d = (E.r0 * (eps * q)) * P.inv_s;
+        return shape_of_tanh(P, tanh_near(E.tp, E.cp, d), tanh_near(E.tm, E.cm, d));
     }
     // g from the shape value f at the point (alcubierre.rs:48-63)
     GEON_D static void g_of_f(const geon_dmetric& P, double f, double* g) {

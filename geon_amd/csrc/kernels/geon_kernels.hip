@@ -475,7 +475,7 @@ extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc(
     // A: the 31 distinct metric points, 4 rounds of the 8 lanes
     Alcubierre::ShapeExp E;
     const bool ex = curv_shape_exp(P);
-    if (ex) Alcubierre::shape_exp(P, Alcubierre::rs(P, x), E);
+    if (ex) Alcubierre::shape_exp(P, Alcubierre::rs2(P, x), E);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int j = t + GEON_CURV_TEAM * r;
@@ -557,7 +557,7 @@ extern "C" __global__ __launch_bounds__(64) void geon_curvature_alc_ricci(
     AlcQuad::base_squares(P, x, sq);
     Alcubierre::ShapeExp E;
     const bool ex = curv_shape_exp(P);
-    if (ex) Alcubierre::shape_exp(P, sqrt(sq[0] + sq[1] + sq[2]), E);
+    if (ex) Alcubierre::shape_exp(P, sq[0] + sq[1] + sq[2], E);
     sfor<8>([&](auto R) { AlcQuad::shape_round<CI(R)>(P, x, sq, t, pt, ex ? &E : nullptr); });
     __syncthreads();
     // B: Γ at x ± h e_x, x ± h e_y (Γ-stencil points 1-4), then x ± h e_z

@@ -65,7 +65,7 @@ extern "C" int hd_curvature_alc_team(const geon_metric_t* m, const double* x, lo
         double sf[AlcStencil::kRow], row[CS::kRow], ric[17];
         Alcubierre::ShapeExp E;
         const bool ex = curv_shape_exp(P);   // the GPU's expansion (GEON_HOST_DEVICE_MATH)
-        if (ex) Alcubierre::shape_exp(P, Alcubierre::rs(P, xk), E);
+        if (ex) Alcubierre::shape_exp(P, Alcubierre::rs2(P, xk), E);
         for (int t = 0; t < GEON_CURV_TEAM; ++t)   // phase A
             for (int r = 0; r < 4; ++r) {
                 const int j = t + GEON_CURV_TEAM * r;
@@ -116,7 +116,7 @@ extern "C" int hd_curvature_alc_quad(const geon_metric_t* m, const double* x, lo
         AlcQuad::base_squares(P, xk, sq);
         Alcubierre::ShapeExp E;
         const bool ex = curv_shape_exp(P);   // the GPU's expansion (GEON_HOST_DEVICE_MATH)
-        if (ex) Alcubierre::shape_exp(P, sqrt(sq[0] + sq[1] + sq[2]), E);
+        if (ex) Alcubierre::shape_exp(P, sq[0] + sq[1] + sq[2], E);
         const Alcubierre::ShapeExp* Ep = ex ? &E : nullptr;
         for (int t = 0; t < 4; ++t) {
             AlcQuad::shape_round<0>(P, xk, sq, t, pt, Ep);

@@ -206,7 +206,7 @@ def test_alcubierre_curvature_expansion_near_oracle(hd_dm, O):
     """The GPU's phase A (both tanh of the shape function expanded about the
     point, Alcubierre::shape_exp) built for the host: the quad form equals
     the team form's Ricci family bit for bit, and T_μν, G, R and Ricci stay
-    within 2e-6 of the largest component of the oracle's (measured 7e-7).
+    within 2e-6 of the largest component of the oracle's (measured 1.0e-6).
     That is the reference's own last-bit noise after two nested central
     differences (h = 1e-5): moving every tanh of the oracle by one ulp
     (or_set_tanh_ulp_shift) moves its T_μν by up to 2.1e-6 of the largest
