@@ -49,7 +49,12 @@ GEON_D void job_ray(const geon_djob& J, unsigned long long local, double* y) {
 // Impact parameter b = L/E of a ray in a static spherically symmetric chart
 // (E = −g_tt p^t, L² = (g_θθ p^θ)² + (g_φφ p^φ)²/sin²θ); rays with
 // |b − crit_b| < fp64_band orbit close to the photon sphere (throat) and
-// are integrated in FP64.
+// are integrated in FP64. So are the rays whose FP32 trace would be handed
+// to FP64 anyway: an ingoing ray below the critical impact parameter has no
+// turning point, so a Schwarzschild ray from outside the photon sphere
+// falls in (every capture, and every dip inside 3.2M, is re-traced in
+// FP64) and a Morris–Thorne ray passes the throat (every dip inside 0.1 b0
+// is); tracing them in FP32 first would only be discarded.
 template <class MP>
 GEON_D bool near_photon_orbit(const geon_dframe& F, const double* y) {
     if (!(F.fp64_band > 0.0)) return false;
@@ -61,7 +66,12 @@ GEON_D bool near_photon_orbit(const geon_dframe& F, const double* y) {
         const double Lt = g[10] * y[6];
         const double Lp = (s != 0.0) ? g[15] * y[7] / s : 0.0;
         const double b = sqrt(Lt * Lt + Lp * Lp) / E;
-        return fabs(b - F.crit_b) < F.fp64_band;
+        if (fabs(b - F.crit_b) < F.fp64_band) return true;
+        if (!(b < F.crit_b - F.fp64_band)) return false;
+        if constexpr (std::is_same<MP, Schwarzschild>::value)
+            return y[1] > 3.0 * F.m.M && y[5] < 0.0;
+        else
+            return y[1] * y[5] < 0.0;
     } else {
         return false;
     }
