@@ -109,6 +109,22 @@ GEON_D double sconst(double c) {
     return c;
 }
 
+// The library's sin and cos, for the arguments the routines below do not
+// reduce (|x| ≥ 2^20: a Payne–Hanek reduction), as a call: inlined, its
+// registers count against every ray kernel's step loop.
+struct SinCos {
+    double s, c;
+};
+#if defined(__HIP_DEVICE_COMPILE__)
+__device__ __attribute__((noinline)) inline SinCos sincos_lib(double x) {
+#else
+inline SinCos sincos_lib(double x) {
+#endif
+    SinCos r;
+    sincos(x, &r.s, &r.c);
+    return r;
+}
+
 // sin x and cos x together, for |x| < 2^20 (else the library sincos): the
 // Cody–Waite reduction x = k·π/2 + r with π/2 split into two doubles and
 // both products taken exactly inside FMAs, then Taylor polynomials of sin to
@@ -120,7 +136,9 @@ GEON_D double sconst(double c) {
 #ifndef GEON_OPCOUNT   // the op-counting build tallies these as one sine and cosine, one tanh
 GEON_D void sincos_cw(double x, double* s, double* c) {
     if (!(fabs(x) < 0x1p20)) {
-        sincos(x, s, c);
+        const SinCos r = sincos_lib(x);
+        *s = r.s;
+        *c = r.c;
         return;
     }
     const double k = rint(x * 0x1.45f306dc9c883p-1);          // x·(2/π)
@@ -168,7 +186,9 @@ GEON_D void sincos_cw(double x, double* s, double* c) {
 // from glibc's than this).
 GEON_D void sincos_acc_dw(double x, double* s, double* sl, double* c, double* cl) {
     if (!(fabs(x) < 0x1p20)) {
-        sincos(x, s, c);
+        const SinCos r = sincos_lib(x);
+        *s = r.s;
+        *c = r.c;
         *sl = 0.0;
         *cl = 0.0;
         return;
@@ -603,8 +623,11 @@ struct Schwarzschild {
     }
     // the same with sin θ and cos θ given (rk4_step hands each stage its
     // values: theta_sincos_dw at the step's θ, stage_sincos_dw at the stages')
+    // FAST: the short sequences whatever the operands, *ok cleared where
+    // their range tests fail (the caller then discards the result).
+    template <bool FAST = false>
     GEON_D static void christoffel_sc(const geon_dmetric& P, const double* x, double sin_t,
-                                      double cos_t, double* c) {
+                                      double cos_t, double* c, bool* ok = nullptr) {
         const double r = x[1], m = P.M;
         const double two_m = 2.0 * m;
         const double r2 = r * r;
@@ -617,7 +640,7 @@ struct Schwarzschild {
         const double ar = fabs(r);
         const bool rok = P.mk && ar > 0x1p-100 && ar < 0x1p100;
         double inv_r, f;
-        if (__builtin_expect(rok, 1)) {
+        if (FAST || __builtin_expect(rok, 1)) {
             inv_r = rcp_rn(r);
             f = 1.0 - div_rn_pz(two_m, r, inv_r);
         } else {
@@ -625,8 +648,10 @@ struct Schwarzschild {
             f = 1.0 - two_m / r;
         }
         const double af = fabs(f);
+        const bool fok = rok && af > 0x1p-100 && af < 0x1p100;
+        if constexpr (FAST) *ok = *ok && fok;
         double gtr, gtt, cot;
-        if (__builtin_expect(rok && af > 0x1p-100 && af < 0x1p100, 1)) {
+        if (FAST || __builtin_expect(fok, 1)) {
             gtr = div_ns(m, r2 * f);
             gtt = div_ns(m * f, r2);
             // cos θ is never zero (π/2 is irrational) nor within 2^-60 of it
@@ -661,6 +686,15 @@ struct Schwarzschild {
                                 double ct) {
         double c[64];
         christoffel_sc(P, y, st, ct, c);
+        contract<Schwarzschild>(c, y + 4, dp);
+    }
+    // accel_sc with the range tests deferred to the end of the step
+    // (rk4_step_full): ok cleared where one fails
+    static constexpr bool fast_stages = true;
+    GEON_D static void accel_sc_fast(const geon_dmetric& P, const double* y, double* dp,
+                                     double st, double ct, bool& ok) {
+        double c[64];
+        christoffel_sc<true>(P, y, st, ct, c, &ok);
         contract<Schwarzschild>(c, y + 4, dp);
     }
     static constexpr bool dep(int d) { return d == 1 || d == 2; }
@@ -703,29 +737,42 @@ struct MorrisThorne {
     // Cofactor inverse of a diagonal matrix: c_ii = Π_{j≠i} m_jj in the
     // cofactor scheme's order, det = m00·c00, g^ii = c_ii/det.
     GEON_D static void ginv(const geon_dmetric&, const double* g, double* gi) {
+        if (__builtin_expect(!ginv_fast(g, gi), 0)) ginv_ieee(g, gi);
+    }
+    // The inverse when 2^-500 < |det| < 2^500 (false otherwise, gi then
+    // meaningless).
+    // g_tt = −1 and g_ll = 1 (g() sets them), so c00 = (1·m10)·m15 and
+    // n5 = (−1·m10)·m15 both round the one product m10·m15 and det = −c00 =
+    // n5 bit for bit: c00/det is exactly −1 and n5/det exactly 1 for every
+    // finite nonzero det. One division serves the other two quotients; with
+    // det ≠ 0 both numerators are nonzero (g_θθ = b0² + l² > 0, g_φφ ≠ 0).
+    GEON_D static bool ginv_fast(const double* g, double* gi) {
+        bool ok = true;
+        ginv_nocheck(g, gi, ok);
+        return ok;
+    }
+    // The short sequence without the branch: ok is cleared where |det| is out
+    // of range, and gi is then meaningless.
+    GEON_D static void ginv_nocheck(const double* g, double* gi, bool& ok) {
+        const double m0 = g[0], m5 = g[5], m10 = g[10], m15 = g[15];
+        const double det = m0 * (m5 * m10 * m15);
+        const double ad = fabs(det);
+        ok = ok && ad > 0x1p-500 && ad < 0x1p500;
+        const double y = rcp_rn(det);
+        gi[0] = -1.0;
+        gi[5] = 1.0;
+        gi[10] = div_rn_pz(m0 * m5 * m15, det, y);
+        gi[15] = div_rn_pz(m0 * m5 * m10, det, y);
+    }
+    // The cofactor quotients as IEEE divisions.
+    GEON_D static void ginv_ieee(const double* g, double* gi) {
         const double m0 = g[0], m5 = g[5], m10 = g[10], m15 = g[15];
         const double c00 = m5 * m10 * m15;
         const double det = m0 * c00;
-        const double n5 = m0 * m10 * m15, n10 = m0 * m5 * m15, n15 = m0 * m5 * m10;
-        const double ad = fabs(det);
-        if (__builtin_expect(ad > 0x1p-500 && ad < 0x1p500, 1)) {
-            // g_tt = −1 and g_ll = 1 (g() sets them), so c00 = (1·m10)·m15 and
-            // n5 = (−1·m10)·m15 both round the one product m10·m15 and det =
-            // −c00 = n5 bit for bit: c00/det is exactly −1 and n5/det exactly
-            // 1 for every finite nonzero det. One division serves the other
-            // two quotients; with det ≠ 0 both numerators are nonzero
-            // (g_θθ = b0² + l² > 0, g_φφ ≠ 0).
-            const double y = rcp_rn(det);
-            gi[0] = -1.0;
-            gi[5] = 1.0;
-            gi[10] = div_rn_pz(n10, det, y);
-            gi[15] = div_rn_pz(n15, det, y);
-        } else {
-            gi[0] = c00 / det;
-            gi[5] = n5 / det;
-            gi[10] = n10 / det;
-            gi[15] = n15 / det;
-        }
+        gi[0] = c00 / det;
+        gi[5] = (m0 * m10 * m15) / det;
+        gi[10] = (m0 * m5 * m15) / det;
+        gi[15] = (m0 * m5 * m10) / det;
     }
     // The generic Γ of metric/mod.rs:51-69 with ∂g by central differences
     // (metric/mod.rs:25-43) written out for this metric: g_θθ = b0² + l² and
@@ -744,17 +791,18 @@ struct MorrisThorne {
         theta_sincos(x[2], &st, &ct);
         christoffel_sc<SIGNED>(P, x, st, ct, c);
     }
-    // the same with st = sin θ and ct = cos θ given
-    template <bool SIGNED = true>
+    // the same with st = sin θ and ct = cos θ given. FAST: the inverse's
+    // short sequence whatever det is, *ok cleared where its range test fails
+    // (the caller then discards the result).
+    template <bool SIGNED = true, bool FAST = false>
     GEON_D static void christoffel_sc(const geon_dmetric& P, const double* x, double st, double ct,
-                                      double* c) {
+                                      double* c, bool* ok = nullptr) {
         const double l = x[1], h = P.fd_h, two_h = P.fd_two_h;
         const double b2 = P.b0 * P.b0;
         const double r2 = b2 + l * l;
         double g0[16], gi[16], dgv[64];
         g0[0] = -1.0; g0[5] = 1.0; g0[10] = r2;
         g0[15] = r2 * st * st;
-        ginv(P, g0, gi);
         const double lp = l + h, lm = l - h;
         const double r2p = b2 + lp * lp, r2m = b2 + lm * lm;
         // quotients by 2h from the exact host reciprocal 1/(2h); metric values
@@ -767,6 +815,11 @@ struct MorrisThorne {
         const double sp = st * P.fd_cos_h + ct * P.fd_sin_h;   // sin(θ + h)
         const double sm = st * P.fd_cos_h - ct * P.fd_sin_h;   // sin(θ − h)
         dgv[2 * 16 + 3 * 4 + 3] = div_rn_pz(r2 * sp * sp - r2 * sm * sm, two_h, i2h);
+        if constexpr (FAST) {
+            ginv_nocheck(g0, gi, *ok);
+        } else {
+            ginv(P, g0, gi);
+        }
         FD<MorrisThorne>::assemble<SIGNED>(gi, dgv, c);
     }
     GEON_D static void accel(const geon_dmetric& P, const double* y, double* dp) {
@@ -781,6 +834,15 @@ struct MorrisThorne {
                                 double ct) {
         double c[64];
         christoffel_sc<false>(P, y, st, ct, c);
+        contract<MorrisThorne>(c, y + 4, dp);
+    }
+    // accel_sc with every range test deferred to the end of the step
+    // (rk4_step_full): ok cleared where one fails
+    static constexpr bool fast_stages = true;
+    GEON_D static void accel_sc_fast(const geon_dmetric& P, const double* y, double* dp,
+                                     double st, double ct, bool& ok) {
+        double c[64];
+        christoffel_sc<false, true>(P, y, st, ct, c, &ok);
         contract<MorrisThorne>(c, y + 4, dp);
     }
     static constexpr bool dep(int d) { return d == 1 || d == 2; }
@@ -1021,6 +1083,12 @@ struct ThetaStages : std::false_type {};
 template <class MP>
 struct ThetaStages<MP, std::void_t<decltype(MP::theta_stages)>>
     : std::integral_constant<bool, MP::theta_stages> {};
+// Metrics whose stages can defer their quotients' range tests (accel_sc_fast)
+template <class MP, class = void>
+struct FastStages : std::false_type {};
+template <class MP>
+struct FastStages<MP, std::void_t<decltype(MP::fast_stages)>>
+    : std::integral_constant<bool, MP::fast_stages> {};
 
 // Steps of a ray still known to be flat from its last flatness test
 // (metrics whose flat regions are tested per step); 0: test again.
@@ -1029,12 +1097,23 @@ struct FlatRun {
     bool nf = false;   // the flatness test of this step already ran and failed
 };
 
-// The full step (Γ evaluated at the four stages).
-template <class MP>
+// The full step (Γ evaluated at the four stages). FAST (metrics with
+// accel_sc_fast): the stages take the short quotient sequences without their
+// range tests and collect the tests' outcome; a lane where one failed takes
+// the step again with the tested stages, so the step's value is the same
+// either way. A branch around each quotient in every stage made the
+// compiler keep copies of the state and of the merged values for the cold
+// side; one branch per step leaves the stages straight-line code.
+template <class MP, bool FAST = FastStages<MP>::value>
 GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, double* out) {
     double acc[8], yt[8], dp[4];
     const double hh = 0.5 * h;
     double s0 = 0.0, c0 = 0.0, s0l = 0.0, c0l = 0.0;
+    bool ok = true;
+    auto accel = [&](const double* ys, double st, double ct) {
+        if constexpr (FAST) MP::accel_sc_fast(P, ys, dp, st, ct, ok);
+        else if constexpr (ThetaStages<MP>::value) MP::accel_sc(P, ys, dp, st, ct);
+    };
     auto stage = [&](const double* ys) {
         if constexpr (ThetaStages<MP>::value) {
             double st, ct;
@@ -1042,7 +1121,7 @@ GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, doub
                 stage_sincos_dw(ys[2], y[2], s0, s0l, c0, c0l, &st, &ct);
             else
                 stage_sincos(ys[2], y[2], s0, c0, &st, &ct);
-            MP::accel_sc(P, ys, dp, st, ct);
+            accel(ys, st, ct);
         } else {
             MP::accel(P, ys, dp);
         }
@@ -1052,7 +1131,7 @@ GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, doub
             theta_sincos_dw(y[2], &s0, &s0l, &c0, &c0l);
         else
             theta_sincos(y[2], &s0, &c0);
-        MP::accel_sc(P, y, dp, s0, c0);
+        accel(y, s0, c0);
     } else {
         MP::accel(P, y, dp);
     }
@@ -1090,6 +1169,9 @@ GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, doub
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
+    if constexpr (FAST) {
+        if (__builtin_expect(!ok, 0)) rk4_step_full<MP, false>(P, y, h, out);
+    }
 }
 
 // Returns true when the step evaluated Γ at its four stages, false when it

@@ -181,6 +181,7 @@ static geon_dmetric metric(int kind) {
     P.ship_x = 0.0;
     P.denom = 2.0 * ::tanh(16.0);
     P.flat_rs = 2.0 + 22.0 * 0.125 + 2e-5;
+    P.mk = 1;   // as make_dmetric sets it for these parameters
     return P;
 }
 
