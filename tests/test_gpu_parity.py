@@ -103,6 +103,56 @@ def test_trace_rays_generic_metrics(ctx, O, kind, kw, obs, esc_r, max_steps):
     assert rel.max() < 1e-8
 
 
+@pytest.mark.parametrize("e", [-300, -110, 110, 300])
+def test_schwarzschild_mass_scale_takes_tested_steps(ctx, O, e):
+    """M = 2^e outside [2^-100, 2^100]: every stage's quotient range test fails,
+    so every step is taken again with the tested stages (IEEE divisions).
+    Scaling M, t, r and the step by 2^e maps the reference's discrete map onto
+    itself exactly — every operation of a Schwarzschild step either scales by
+    a power of two or is scale-free, and nothing leaves the normal range for
+    |e| ≤ 300 — and the oracle shows it on this camera (the same classes, step
+    counts, directions and scaled final states at every e). So the GPU must
+    give its own M = 1 results bit for bit, through the other code path, and
+    meet the oracle's bar."""
+    import geon_amd as geon
+    s = 2.0 ** e
+    obs1 = (0.0, 30.0, math.pi / 2, 0.0)
+    om1 = O.metric(O.SCHWARZSCHILD, mass=1.0, horizon_pad=1.005, escape_radius=100.0)
+    y01 = O.camera_rays(om1, O.camera(obs1, 60.0, 64, 48))
+    g1 = geon.trace_rays(om1, y01, 0.05, 4000, ctx=ctx)
+    om = O.metric(O.SCHWARZSCHILD, mass=s, horizon_pad=1.005, escape_radius=100.0)
+    y0 = O.camera_rays(om, O.camera((0.0, 30.0 * s, math.pi / 2, 0.0), 60.0, 64, 48))
+    scale = np.array([s, s, 1.0, 1.0, 1.0, 1.0, 1.0 / s, 1.0 / s])
+    assert np.array_equal(y0, y01 * scale)
+    got = geon.trace_rays(om, y0, 0.05 * s, 4000, ctx=ctx)
+    assert np.array_equal(got["cls"], g1["cls"])
+    assert np.array_equal(got["steps"], g1["steps"])
+    assert np.array_equal(got["dir"], g1["dir"], equal_nan=True)
+    assert np.array_equal(got["y"], g1["y"] * scale, equal_nan=True)
+    assert np.array_equal(got["resid"], g1["resid"], equal_nan=True)
+    ref = O.trace_rays(om, y0, 0.05 * s, 4000)
+    compare_rays(O, om, y0, got, ref, 0.05 * s, 4000, f"Schwarzschild M=2^{e}", esc_r=100.0 * s)
+
+
+@pytest.mark.parametrize("e", [-130, 130])
+def test_morris_thorne_tested_inverse(ctx, O, e):
+    """b0 = 2^±130 puts |det g| = (b0² + l²)² sin²θ outside [2^-500, 2^500]
+    (about 2^-504 and 2^535 on this camera), so every Morris–Thorne step is
+    taken again with the tested inverse (IEEE divisions). The finite
+    differences keep their absolute h = 1e-5, so this is not a rescaled
+    frame; it is held to the oracle's bar on the same discrete map."""
+    import geon_amd as geon
+    s = 2.0 ** e
+    om = O.metric(O.MORRIS_THORNE, throat_radius=s, escape_radius=60.0 * s)
+    y0 = O.camera_rays(om, O.camera((0.0, 15.0 * s, math.pi / 2, 0.0), 60.0, 64, 48))
+    det = np.prod(np.diag(O.g(om, y0[0][:4])))
+    assert not (2.0 ** -500 < abs(det) < 2.0 ** 500)
+    ref = O.trace_rays(om, y0, 0.05 * s, 3000)
+    got = geon.trace_rays(om, y0, 0.05 * s, 3000, ctx=ctx)
+    compare_rays(O, om, y0, got, ref, 0.05 * s, 3000, f"Morris-Thorne b0=2^{e}",
+                 esc_r=60.0 * s)
+
+
 def test_superluminal_alcubierre_conditioning(ctx, O):
     """The superluminal bubble (v = 2, s = 0.5) has horizons in its walls: there
     the reference's own result is ill-conditioned in the last bit of tanh.

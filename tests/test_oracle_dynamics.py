@@ -154,3 +154,29 @@ def test_trace_rays_matches_single_trace(O):
         assert np.array_equal(y, res["y"][i]) and resid == res["resid"][i]
         if cls == O.ESCAPED:
             assert np.array_equal(res["dir"][i], O.asymptotic_direction(m, y[:4], y[4:]))
+
+
+@pytest.mark.parametrize("e", [-300, -110, 110, 300])
+def test_schwarzschild_step_is_scale_free(O, e):
+    """Scaling M, t, r and the step by 2^e maps the Schwarzschild discrete map
+    onto itself exactly: every Γ entry carries the power of M its dimension
+    says (M/(r²f), Mf/r², r − 2M, 1/r, cot θ), so each rounded operation of a
+    step scales by a power of two or not at all, and |e| ≤ 300 keeps every
+    value normal. Classes, step counts, directions and the scaled final
+    states must equal the M = 1 trace bit for bit. A term with a wrong power
+    of r or M (M/r for M/r², r² for r − 2M) changes the trajectories at
+    some e; the GPU test of the tested-division steps builds on this."""
+    runs = []
+    for k in (0, e):
+        s = 2.0 ** k
+        m = O.metric(O.SCHWARZSCHILD, mass=s, horizon_pad=1.005, escape_radius=100.0)
+        y0 = O.camera_rays(m, O.camera((0.0, 30.0 * s, math.pi / 2, 0.0), 60.0, 24, 18))
+        runs.append((s, y0, O.trace_rays(m, y0, 0.05 * s, 4000)))
+    (_, y01, r1), (s, y0, r) = runs
+    scale = np.array([s, s, 1.0, 1.0, 1.0, 1.0, 1.0 / s, 1.0 / s])
+    assert np.array_equal(y0, y01 * scale)
+    assert np.bincount(r1["cls"], minlength=4)[1] > 0 and np.bincount(r1["cls"], minlength=4)[2] > 0
+    assert np.array_equal(r["cls"], r1["cls"])
+    assert np.array_equal(r["steps"], r1["steps"])
+    assert np.array_equal(r["dir"], r1["dir"], equal_nan=True)
+    assert np.array_equal(r["y"], r1["y"] * scale, equal_nan=True)
