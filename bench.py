@@ -91,6 +91,8 @@ def parse():
     ap.add_argument("--copy-probe", action="store_true",
                     help="one 1 GiB device-to-device copy (HBM counter calibration) and exit")
     ap.add_argument("--serving-seconds", type=float, default=2.0)
+    ap.add_argument("--quantum-steps", type=int, default=16384,
+                    help="serving: steps per lane in each quantum")
     return ap.parse_args()
 
 
@@ -214,7 +216,8 @@ def main():
         result["ragged_batch"] = ragged_batch(geon, ctx, torch, stream, ev)
         log(f"ragged batch: {json.dumps(result['ragged_batch'])}")
     if "serving" in sections:
-        result["serving"] = serving(geon, ctx, torch, stream, outs, ev, args.serving_seconds)
+        result["serving"] = serving(geon, ctx, torch, stream, outs, ev, args.serving_seconds,
+                                    args.quantum_steps)
         log(f"serving: {json.dumps(result['serving'])}")
     if "curvature" in sections:
         result["curvature"] = curvature(geon, ctx, torch, stream, ev, alc, mt)
@@ -428,7 +431,7 @@ def ragged_batch(geon, ctx, torch, stream, ev):
     return res
 
 
-def serving(geon, ctx, torch, stream, outs, ev, seconds):
+def serving(geon, ctx, torch, stream, outs, ev, seconds, quantum=16384):
     """Poisson request streams of `seconds` of arrivals at each rate. At each rate
     the plain FP64 kernel then renders the same requests one after another:
     `speedup_vs_plain_fp64` is that FP64 device time over the server's wall
@@ -440,7 +443,8 @@ def serving(geon, ctx, torch, stream, outs, ev, seconds):
     out = {"workload": f"Poisson arrivals for {seconds} s at each rate: 80% renders side "
                        "log-uniform 32..2048 (random metric, camera r∈[10,50], escape 100, max 5000, "
                        "FP32-mixed), 20% bisections 64 rays × 40 halvings (M=1, r_obs∈[10,50], "
-                       "photon-sphere decisions); seed 7; 8 GiB admission budget, 2048-step quanta",
+                       "photon-sphere decisions); seed 7; 8 GiB admission budget, "
+                       f"{quantum}-step quanta",
            "rates": {}}
     tune_kinds(geon, ctx)
     for rate in (50, 100, 200, 400):
@@ -448,7 +452,7 @@ def serving(geon, ctx, torch, stream, outs, ev, seconds):
         rng = np.random.default_rng(rate)
         renders = [i for i, r in enumerate(reqs) if r["kind"] == "render"]
         keep = set(rng.choice(renders, min(16, len(renders)), replace=False).tolist())
-        srv = geon.Server(ctx, hbm_budget=8 << 30, quantum_steps=2048)
+        srv = geon.Server(ctx, hbm_budget=8 << 30, quantum_steps=quantum)
         tickets = []
         t_start = time.perf_counter()
         i = 0

@@ -794,7 +794,7 @@ int geon_serve_create(geon_ctx_t* ctx, const geon_serve_cfg_t* cfg, geon_server_
     memset(&s->cfg, 0, sizeof s->cfg);
     if (cfg) s->cfg = *cfg;
     if (!s->cfg.hbm_budget) s->cfg.hbm_budget = 8ull << 30;
-    if (!s->cfg.quantum_steps) s->cfg.quantum_steps = 2048;
+    if (!s->cfg.quantum_steps) s->cfg.quantum_steps = 16384;
     if (!s->cfg.max_jobs) s->cfg.max_jobs = 16384;
     int st = GEON_OK;
     auto bail = [&](int code) {

@@ -247,11 +247,17 @@ GEON_D void sincos_acc(double x, double* s, double* c) {
 // (the same for cos), i.e. about as close to the correctly rounded value as
 // sincos_acc's except where θ passes within |d| of a zero of the function,
 // where the error is that absolute amount. Larger |d| take sincos_acc.
+// FB and ok as sincos_rot.
+template <bool FB = true>
 GEON_D void sincos_rot_dw(double th, double s0, double s0l, double c0, double c0l, double d,
-                          double* s, double* c) {
-    if (!(fabs(d) <= 0.0625)) {
-        sincos_acc(th, s, c);
-        return;
+                          double* s, double* c, bool* ok = nullptr) {
+    if constexpr (FB) {
+        if (!(fabs(d) <= 0.0625)) {
+            sincos_acc(th, s, c);
+            return;
+        }
+    } else {
+        *ok = *ok && fabs(d) <= 0.0625;
     }
     const double d2 = d * d;
     double ps = __builtin_fma(0x1.71de3a556c734p-19, d2, sconst(-0x1.a01a01a01a01ap-13));  //  1/9!, −1/7!
@@ -337,10 +343,18 @@ __device__ __attribute__((noinline)) inline double tanh_cw_call(double x) { retu
 // sin θ = s0 + (s0·(cos d − 1) + c0·sin d) and cos θ = c0 + (c0·(cos d − 1)
 // − s0·sin d): the corrections are small, so the result carries the error of
 // (s0, c0) plus one rounding. Larger |d| take sincos_cw(θ).
-GEON_D void sincos_rot(double th, double s0, double c0, double d, double* s, double* c) {
-    if (!(fabs(d) <= 0.0625)) {
-        sincos_cw(th, s, c);
-        return;
+// FB = false: no fallback; *ok is cleared for |d| > 1/16 (the caller then
+// discards the result).
+template <bool FB = true>
+GEON_D void sincos_rot(double th, double s0, double c0, double d, double* s, double* c,
+                       bool* ok = nullptr) {
+    if constexpr (FB) {
+        if (!(fabs(d) <= 0.0625)) {
+            sincos_cw(th, s, c);
+            return;
+        }
+    } else {
+        *ok = *ok && fabs(d) <= 0.0625;
     }
     const double d2 = d * d;
     double ps = __builtin_fma(0x1.71de3a556c734p-19, d2, sconst(-0x1.a01a01a01a01ap-13));  //  1/9!, −1/7!
@@ -371,11 +385,13 @@ GEON_D void theta_sincos(double x, double* s, double* c) {
 // sin θ and cos θ of an RK4 stage whose θ differs from the step's θ0 by the
 // stage increment: on the GPU by rotation from (sin θ0, cos θ0); the host
 // build evaluates them directly like theta_sincos.
-GEON_D void stage_sincos(double th, double th0, double s0, double c0, double* s, double* c) {
+template <bool FB = true>
+GEON_D void stage_sincos(double th, double th0, double s0, double c0, double* s, double* c,
+                         bool* ok = nullptr) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    sincos_rot(th, s0, c0, th - th0, s, c);
+    sincos_rot<FB>(th, s0, c0, th - th0, s, c, ok);
 #else
-    (void)th0; (void)s0; (void)c0;
+    (void)th0; (void)s0; (void)c0; (void)ok;
     sincos(th, s, c);
 #endif
 }
@@ -395,12 +411,13 @@ GEON_D void theta_sincos_dw(double x, double* s, double* sl, double* c, double* 
     *cl = 0.0;
 #endif
 }
+template <bool FB = true>
 GEON_D void stage_sincos_dw(double th, double th0, double s0, double s0l, double c0, double c0l,
-                            double* s, double* c) {
+                            double* s, double* c, bool* ok = nullptr) {
 #if defined(__HIP_DEVICE_COMPILE__) || defined(GEON_HOST_DEVICE_MATH)
-    sincos_rot_dw(th, s0, s0l, c0, c0l, th - th0, s, c);
+    sincos_rot_dw<FB>(th, s0, s0l, c0, c0l, th - th0, s, c, ok);
 #else
-    (void)th0; (void)s0; (void)s0l; (void)c0; (void)c0l;
+    (void)th0; (void)s0; (void)s0l; (void)c0; (void)c0l; (void)ok;
     sincos(th, s, c);
 #endif
 }
@@ -624,8 +641,9 @@ struct Schwarzschild {
     // the same with sin θ and cos θ given (rk4_step hands each stage its
     // values: theta_sincos_dw at the step's θ, stage_sincos_dw at the stages')
     // FAST: the short sequences whatever the operands, *ok cleared where
-    // their range tests fail (the caller then discards the result).
-    template <bool FAST = false>
+    // their range tests fail (the caller then discards the result). IEEE:
+    // the IEEE divisions whatever the operands.
+    template <bool FAST = false, bool IEEE = false>
     GEON_D static void christoffel_sc(const geon_dmetric& P, const double* x, double sin_t,
                                       double cos_t, double* c, bool* ok = nullptr) {
         const double r = x[1], m = P.M;
@@ -640,7 +658,7 @@ struct Schwarzschild {
         const double ar = fabs(r);
         const bool rok = P.mk && ar > 0x1p-100 && ar < 0x1p100;
         double inv_r, f;
-        if (FAST || __builtin_expect(rok, 1)) {
+        if (!IEEE && (FAST || __builtin_expect(rok, 1))) {
             inv_r = rcp_rn(r);
             f = 1.0 - div_rn_pz(two_m, r, inv_r);
         } else {
@@ -651,7 +669,7 @@ struct Schwarzschild {
         const bool fok = rok && af > 0x1p-100 && af < 0x1p100;
         if constexpr (FAST) *ok = *ok && fok;
         double gtr, gtt, cot;
-        if (FAST || __builtin_expect(fok, 1)) {
+        if (!IEEE && (FAST || __builtin_expect(fok, 1))) {
             gtr = div_ns(m, r2 * f);
             gtt = div_ns(m * f, r2);
             // cos θ is never zero (π/2 is irrational) nor within 2^-60 of it
@@ -695,6 +713,14 @@ struct Schwarzschild {
                                      double st, double ct, bool& ok) {
         double c[64];
         christoffel_sc<true>(P, y, st, ct, c, &ok);
+        contract<Schwarzschild>(c, y + 4, dp);
+    }
+    // accel_sc with every quotient an IEEE division (the step taken again)
+    // accel_sc with every quotient an IEEE division (the step taken again)
+    GEON_D static void accel_sc_ieee(const geon_dmetric& P, const double* y, double* dp,
+                                     double st, double ct) {
+        double c[64];
+        christoffel_sc<false, true>(P, y, st, ct, c);
         contract<Schwarzschild>(c, y + 4, dp);
     }
     static constexpr bool dep(int d) { return d == 1 || d == 2; }
@@ -793,8 +819,9 @@ struct MorrisThorne {
     }
     // the same with st = sin θ and ct = cos θ given. FAST: the inverse's
     // short sequence whatever det is, *ok cleared where its range test fails
-    // (the caller then discards the result).
-    template <bool SIGNED = true, bool FAST = false>
+    // (the caller then discards the result). IEEE: the inverse's IEEE
+    // divisions whatever det is.
+    template <bool SIGNED = true, bool FAST = false, bool IEEE = false>
     GEON_D static void christoffel_sc(const geon_dmetric& P, const double* x, double st, double ct,
                                       double* c, bool* ok = nullptr) {
         const double l = x[1], h = P.fd_h, two_h = P.fd_two_h;
@@ -817,6 +844,8 @@ struct MorrisThorne {
         dgv[2 * 16 + 3 * 4 + 3] = div_rn_pz(r2 * sp * sp - r2 * sm * sm, two_h, i2h);
         if constexpr (FAST) {
             ginv_nocheck(g0, gi, *ok);
+        } else if constexpr (IEEE) {
+            ginv_ieee(g0, gi);
         } else {
             ginv(P, g0, gi);
         }
@@ -843,6 +872,13 @@ struct MorrisThorne {
                                      double st, double ct, bool& ok) {
         double c[64];
         christoffel_sc<false, true>(P, y, st, ct, c, &ok);
+        contract<MorrisThorne>(c, y + 4, dp);
+    }
+    // accel_sc with every quotient an IEEE division (the step taken again)
+    GEON_D static void accel_sc_ieee(const geon_dmetric& P, const double* y, double* dp,
+                                     double st, double ct) {
+        double c[64];
+        christoffel_sc<false, false, true>(P, y, st, ct, c);
         contract<MorrisThorne>(c, y + 4, dp);
     }
     static constexpr bool dep(int d) { return d == 1 || d == 2; }
@@ -1099,12 +1135,17 @@ struct FlatRun {
 
 // The full step (Γ evaluated at the four stages). FAST (metrics with
 // accel_sc_fast): the stages take the short quotient sequences without their
-// range tests and collect the tests' outcome; a lane where one failed takes
-// the step again with the tested stages, so the step's value is the same
-// either way. A branch around each quotient in every stage made the
-// compiler keep copies of the state and of the merged values for the cold
-// side; one branch per step leaves the stages straight-line code.
-template <class MP, bool FAST = FastStages<MP>::value>
+// range tests, and the stage rotation of (sin θ, cos θ) without its fallback
+// for increments beyond 1/16, collecting the tests' outcome; a lane where
+// one failed takes the step again (IEEE: every quotient an IEEE division,
+// the rotation with its fallback). The short sequences equal the IEEE
+// quotients wherever their tests pass, so the step's value is the one the
+// tested stages give either way. A branch around each quotient in every
+// stage made the compiler keep copies of the state and of the merged values
+// for the cold side; one branch per step leaves the stages straight-line
+// code, and without the rotation's fallback inside them the Morris–Thorne
+// step loop fits four waves per SIMD.
+template <class MP, bool FAST = FastStages<MP>::value, bool IEEE = false>
 GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, double* out) {
     double acc[8], yt[8], dp[4];
     const double hh = 0.5 * h;
@@ -1112,15 +1153,16 @@ GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, doub
     bool ok = true;
     auto accel = [&](const double* ys, double st, double ct) {
         if constexpr (FAST) MP::accel_sc_fast(P, ys, dp, st, ct, ok);
+        else if constexpr (IEEE) MP::accel_sc_ieee(P, ys, dp, st, ct);
         else if constexpr (ThetaStages<MP>::value) MP::accel_sc(P, ys, dp, st, ct);
     };
     auto stage = [&](const double* ys) {
         if constexpr (ThetaStages<MP>::value) {
             double st, ct;
             if constexpr (MP::theta_dw)
-                stage_sincos_dw(ys[2], y[2], s0, s0l, c0, c0l, &st, &ct);
+                stage_sincos_dw<!FAST>(ys[2], y[2], s0, s0l, c0, c0l, &st, &ct, &ok);
             else
-                stage_sincos(ys[2], y[2], s0, c0, &st, &ct);
+                stage_sincos<!FAST>(ys[2], y[2], s0, c0, &st, &ct, &ok);
             accel(ys, st, ct);
         } else {
             MP::accel(P, ys, dp);
@@ -1170,7 +1212,7 @@ GEON_D void rk4_step_full(const geon_dmetric& P, const double* y, double h, doub
 #pragma unroll
     for (int i = 0; i < 8; ++i) out[i] = y[i] + sixth * acc[i];
     if constexpr (FAST) {
-        if (__builtin_expect(!ok, 0)) rk4_step_full<MP, false>(P, y, h, out);
+        if (__builtin_expect(!ok, 0)) rk4_step_full<MP, false, true>(P, y, h, out);
     }
 }
 

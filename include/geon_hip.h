@@ -359,7 +359,7 @@ int  geon_autotune(geon_ctx_t* ctx, int32_t metric_kind, int32_t precision,
  * the rays already in flight. */
 typedef struct geon_serve_cfg {
     uint64_t hbm_budget;       /* bytes of request buffers in flight (0: 8 GiB)       */
-    uint32_t quantum_steps;    /* RK4 steps per lane per quantum (0: 2048)            */
+    uint32_t quantum_steps;    /* RK4 steps per lane per quantum (0: 16384)           */
     uint32_t max_jobs;         /* job-table capacity over the server's life (0: 16384) */
 } geon_serve_cfg_t;
 
